@@ -1,0 +1,171 @@
+/*
+ * m25x.h - the C ABI of libm25x.so, the MI355X (gfx950) kernels of the per-frequency MoM solve.
+ *
+ * Plain pointers, sizes and a hipStream_t passed as void*; no torch or HIP types in the signatures. Every
+ * pointer named "device" is HIP device memory; every function is asynchronous on the given stream unless it
+ * says otherwise. Return value: 0, or a negative M25X_E* code with a message in m25x_last_error() (per thread).
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the reference root). The
+ * reference binds those through pybind11 (src/mom25d/core/mom25d_core.cpp:1658-1702) or through the GPU backend
+ * contract (src/mom25d/gpu.py:28-48); the ctypes binding of this header is mom25d_amd/_lib.py.
+ *
+ * Complex numbers are interleaved (re, im): complex128 as two doubles, complex64 as two floats. Matrices are
+ * row-major with a leading dimension (the reference's C-ordered Z), which is the column-major storage of Z^T:
+ * the LU routines factor that column-major A = Z^T in place exactly as scipy.linalg.lu_factor(Z.T) lays it out.
+ */
+#ifndef M25X_H
+#define M25X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define M25X_OK 0
+#define M25X_EINVAL (-1)    /* bad argument (the reference's std::invalid_argument -> ValueError) */
+#define M25X_EHIP (-2)      /* a HIP runtime error (RuntimeError) */
+#define M25X_ENOMEM (-3)    /* device allocation failed (MemoryError) */
+#define M25X_ESINGULAR (-4) /* the LU met an exactly zero pivot (LAPACK info > 0) */
+
+#define M25X_C128 0
+#define M25X_C64 1
+
+/* Green's tables (device): AQ holds ntab x nfine points of (Re A, Im A, Re Q, Im Q); per table dr, d, dimg
+ * (double) and cA, cq, cAi, cqi (complex128); tab (L x L int32: the table of an interface pair), kind (ntab int32:
+ * 0 same interface, 1 cross extracted, 2 cross plain). The reference's tb dict (src/mom25d/fast.py:556-568). */
+typedef struct m25x_tables {
+    int ntab, nfine, L;
+    const double *AQ;
+    const double *dr, *d, *dimg;
+    const double *cA, *cq, *cAi, *cqi;
+    const int *tab, *kind;
+} m25x_tables;
+
+/* Mesh and RWG basis (device): the reference's g dict (src/mom25d/fast.py:389-398). ep/em: T+ and T- of each
+ * edge; lt: triangle layer index; te: Nt x 3 edge of each triangle slot or -1; ta: Nt x 3 signed alpha; tv:
+ * Nt x 3 x 2 free vertex; ap/am: signed alpha per edge; vp/vm: Ne x 2; cent: Nt x 2; area: Nt. */
+typedef struct m25x_mesh {
+    int Ne, Nt;
+    const int *ep, *em, *lt, *te;
+    const double *ap, *am, *vp, *vm, *cent, *area, *ta, *tv;
+} m25x_mesh;
+
+/* Quadrature geometry (device): the reference's q dict (src/mom25d/fast.py:399-400): nodes (nn x 2), tris
+ * (Nt x 3 int32), P3 (Nt x 3 x 2), W3 (Nt x 3), P6 (Nt x 6 x 2), W6 (Nt x 6). */
+typedef struct m25x_quad {
+    const double *nodes;
+    const int *tris;
+    const double *P3, *W3, *P6, *W6;
+} m25x_quad;
+
+const char *m25x_last_error(void);
+int m25x_version(void);
+/* Makes `device` current and caches its properties; called once per device before the others. */
+int m25x_init(int device);
+
+/* ---- fill ---------------------------------------------------------------------------------------------- */
+
+/* Far-field (centroid-rule) entries of every edge pair into the Ne x Ne edge block of Z (row-major, leading
+ * dimension ld, complex128 or complex64 per zdtype), symmetric. Replaces fill_far / fill_far_c64
+ * (src/mom25d/core/mom25d_core.cpp:132-173). */
+int m25x_fill_far(void *Z, int zdtype, long ld, const m25x_mesh *g, const m25x_tables *tb, double w, void *stream);
+
+/* Centroid-rule entries of row x column edge lists into out (nr x nc complex128, row-major). Replaces far_block
+ * (src/mom25d/core/mom25d_core.cpp:176-211). */
+int m25x_far_block(const int *rows, int nr, const int *cols, int nc, const m25x_mesh *g, const m25x_tables *tb,
+                   double w, double *out, void *stream);
+
+/* The seven accurate-minus-centroid moments (M0, Mox, Moy, Msx, Msy, Mrr, Mq; complex128) of each near pair into
+ * M (n x 7 complex128) and Mq alone into qcorr (n complex128, may be NULL). stat (n x 12 doubles, may be NULL):
+ * the pairs' analytic (extracted-part) moments, written when stat_ready is 0 and read when it is 1. The
+ * per-pair half of near_core (src/mom25d/core/mom25d_core.cpp:417-476). */
+int m25x_near_moments(const m25x_mesh *g, const m25x_tables *tb, const m25x_quad *q, const int *P, const int *Q,
+                      long n, double w, double *stat, int stat_ready, double *M, double *qcorr, void *stream);
+
+/* Z[erow[e], ecol[e]] += sum over the contributions c of entry e (contrib[estart[e] .. estart[e+1]), each
+ * 9 * pair + 3 * s + u) of the pair's moments combined for its edge slots s, u; deterministic, no atomics. The
+ * scatter half of near_into_dense (src/mom25d/core/mom25d_core.cpp:477-507, 520-537). */
+int m25x_near_scatter(void *Z, int zdtype, long ld, const m25x_mesh *g, const int *P, const int *Q,
+                      const double *M, long nentry, const int *erow, const int *ecol, const int *estart,
+                      const int *contrib, double w, void *stream);
+
+/* 9 COO triplets a pair (ri, ci int32; vv complex128), row -1 for unused slots. Replaces near_coo's output
+ * (src/mom25d/core/mom25d_core.cpp:559-575) from the moments of m25x_near_moments. */
+int m25x_near_coo(const m25x_mesh *g, const int *P, const int *Q, const double *M, long n, double w, int *ri,
+                  int *ci, double *vv, void *stream);
+
+/* Near triangle pairs rho^2 + dz^2 < (fac max(size_p, size_q))^2, sorted by (p, q). Two calls: with P == NULL
+ * the count is written to *count (synchronous); then the pairs (int32) are written. Replaces near_pairs
+ * (src/mom25d/core/mom25d_core.cpp:1216-1263). counts: Nt + 1 int64 of device scratch. */
+int m25x_near_pairs(const double *cent, const double *size, const double *z, int Nt, double fac, long long *counts,
+                    long *count, int *P, int *Q, void *stream);
+
+/* G_q x area_t x area_f between every triangle t and the fan triangles F (Nt x nf complex128). Replaces fan_q
+ * (src/mom25d/core/mom25d_core.cpp:1585-1603). */
+int m25x_fan_q(const m25x_mesh *g, const m25x_tables *tb, const int *F, int nf, double *out, void *stream);
+
+/* out[t, v] = sum_q D(|P[t, q] - V[v]|) W[t, q] over a radial table D (n complex128, step dr), linear lookup.
+ * Replaces radial_table_sum (src/mom25d/core/mom25d_core.cpp:1269-1298). */
+int m25x_radial_table_sum(const double *px, const double *py, const double *W, long nt, long nq, const double *vx,
+                          const double *vy, long nv, const double *table, long n, double dr, double *out,
+                          void *stream);
+
+/* out[s, i] = sum_k J0(rho_i kr_k) W[s, k] over a Sommerfeld contour, kr complex (nk complex128), W (ns x nk
+ * complex128), out (ns x nr complex128): real nodes through the real J0, lifted ones through the complex series
+ * / asymptotic expansion. Replaces hankel_real + the lifted part of j0_sum (src/mom25d/core/mom25d_core.cpp:
+ * 1430-1454; src/mom25d/via.py:90-101). */
+int m25x_hankel(const double *rho, int nr, const double *kr, int nk, const double *W, int ns, double *out,
+                void *stream);
+
+/* J_n(z), complex z, n = 0, 1, 2, elementwise. Replaces bessel_jn_complex
+ * (src/mom25d/core/mom25d_core.cpp:1510-1558). */
+int m25x_bessel_jn_complex(int n, const double *z, long total, double *out, void *stream);
+
+/* Z <- diag(d) Z diag(d) in place (n x n complex128, leading dimension ld) and, when Z64 is not NULL, its
+ * complex64 copy (leading dimension ld64). Replaces scale_c64 and _equilibrate
+ * (src/mom25d/core/mom25d_core.cpp:1563-1580; src/mom25d/fast.py:294-305). */
+int m25x_scale_c64(double *Z, long n, long ld, const double *d, float *Z64, long ld64, void *stream);
+
+/* d[i] = 1 / sqrt(|Z_ii| + 1e-300) (the equilibration scales, src/mom25d/fast.py:799). */
+int m25x_diag_scales(const double *Z, long n, long ld, double *d, void *stream);
+
+/* ---- dense complex LU ---------------------------------------------------------------------------------- */
+
+/* Device workspace bytes m25x_getrf needs for an n x n matrix of the given dtype. */
+size_t m25x_getrf_work_bytes(int dtype, long n);
+
+/* LU with partial pivoting of the column-major n x n matrix A (leading dimension lda; the row-major Z seen as
+ * Z^T), in place: L unit lower, U upper, ipiv 0-based LAPACK interchanges (int32, device), the layout of
+ * scipy.linalg.lu_factor(Z.T). complex128 (trailing updates on FP64 MFMA) or complex64 (FP32 MFMA, FP32
+ * accumulation). info (device int, may be NULL): 0, or 1 + the first zero pivot. Replaces the GPU backend's
+ * factor (src/mom25d/gpu.py:69-74; src/mom25d/gpu_vk.py:515-633) and LAPACK zgetrf/cgetrf
+ * (src/mom25d/fast.py:318-320). */
+int m25x_getrf(int dtype, void *A, long n, long lda, int *ipiv, int *info, void *work, size_t work_bytes,
+               void *stream);
+
+/* The interchanges of ipiv as one permutation (perm = ipiv's swaps applied in order to 0..n-1), once per factor:
+ * P^T b = b[perm], P z scatters z[i] to perm[i]. */
+int m25x_ipiv_perm(const int *ipiv, long n, int *perm, void *stream);
+
+/* Device workspace bytes m25x_getrs needs. */
+size_t m25x_getrs_work_bytes(int dtype, long n, long ldb);
+
+/* Solves with the factor of m25x_getrf: trans = 1: Z X = B (A^T X = B, lu_solve(..., trans=1)), trans = 0:
+ * A X = B. B: n x nrhs row-major (leading dimension ldb >= nrhs, 1 <= nrhs <= 8), the factor's dtype, overwritten
+ * with X. perm: m25x_ipiv_perm's (or NULL: computed here). Replaces the backend's solve (src/mom25d/gpu.py:76-80;
+ * src/mom25d/gpu_vk.py:650-678) and zgetrs / cgetrs (src/mom25d/fast.py:356-358). */
+int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B, long ldb,
+               int nrhs, int trans, void *work, size_t work_bytes, void *stream);
+
+/* ---- refinement products ------------------------------------------------------------------------------- */
+
+/* Y = Z X: Z n x n complex128 row-major (leading dimension ld), X and Y n x k complex128 row-major (k <= 16).
+ * The FP64 products of the GMRES refinement (src/mom25d/fast.py:1004-1007, A = Z @ V). */
+int m25x_zgemm_nk(const double *Z, long n, long ld, const double *X, int k, double *Y, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,104 @@
+"""ctypes binding of libm25x.so (include/m25x.h). Loading fails loudly: there is no CPU fallback."""
+import ctypes as C
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libm25x.so')
+
+C128, C64 = 0, 1
+EINVAL, EHIP, ENOMEM, ESINGULAR = -1, -2, -3, -4
+
+
+class Tables(C.Structure):
+    _fields_ = [('ntab', C.c_int), ('nfine', C.c_int), ('L', C.c_int), ('AQ', C.c_void_p),
+                ('dr', C.c_void_p), ('d', C.c_void_p), ('dimg', C.c_void_p),
+                ('cA', C.c_void_p), ('cq', C.c_void_p), ('cAi', C.c_void_p), ('cqi', C.c_void_p),
+                ('tab', C.c_void_p), ('kind', C.c_void_p)]
+
+
+class Mesh(C.Structure):
+    _fields_ = [('Ne', C.c_int), ('Nt', C.c_int), ('ep', C.c_void_p), ('em', C.c_void_p), ('lt', C.c_void_p),
+                ('te', C.c_void_p), ('ap', C.c_void_p), ('am', C.c_void_p), ('vp', C.c_void_p), ('vm', C.c_void_p),
+                ('cent', C.c_void_p), ('area', C.c_void_p), ('ta', C.c_void_p), ('tv', C.c_void_p)]
+
+
+class Quad(C.Structure):
+    _fields_ = [('nodes', C.c_void_p), ('tris', C.c_void_p), ('P3', C.c_void_p), ('W3', C.c_void_p),
+                ('P6', C.c_void_p), ('W6', C.c_void_p)]
+
+
+P, I, L, D, S = C.c_void_p, C.c_int, C.c_long, C.c_double, C.c_size_t
+_SIGS = {
+    'm25x_version': ([], I),
+    'm25x_init': ([I], I),
+    'm25x_fill_far': ([P, I, L, P, P, D, P], I),
+    'm25x_far_block': ([P, I, P, I, P, P, D, P, P], I),
+    'm25x_near_moments': ([P, P, P, P, P, L, D, P, I, P, P, P], I),
+    'm25x_near_scatter': ([P, I, L, P, P, P, P, L, P, P, P, P, D, P], I),
+    'm25x_near_coo': ([P, P, P, P, L, D, P, P, P, P], I),
+    'm25x_near_pairs': ([P, P, P, I, D, P, C.POINTER(C.c_long), P, P, P], I),
+    'm25x_fan_q': ([P, P, P, I, P, P], I),
+    'm25x_radial_table_sum': ([P, P, P, L, L, P, P, L, P, L, D, P, P], I),
+    'm25x_hankel': ([P, I, P, I, P, I, P, P], I),
+    'm25x_bessel_jn_complex': ([I, P, L, P, P], I),
+    'm25x_scale_c64': ([P, L, L, P, P, L, P], I),
+    'm25x_diag_scales': ([P, L, L, P, P], I),
+    'm25x_getrf_work_bytes': ([I, L], S),
+    'm25x_getrf': ([I, P, L, L, P, P, P, S, P], I),
+    'm25x_ipiv_perm': ([P, L, P, P], I),
+    'm25x_getrs_work_bytes': ([I, L, L], S),
+    'm25x_getrs': ([I, P, L, L, P, P, P, L, I, I, P, S, P], I),
+    'm25x_zgemm_nk': ([P, L, L, P, I, P, P], I),
+}
+
+_lib = None
+_lock = threading.Lock()
+_inited = set()
+
+
+class M25xError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded library (RuntimeError when it is not built: `python -c 'import __graft_entry__ as g; g.build()'`
+    or `make -C mom25d_amd/csrc`)."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not os.path.exists(LIB_PATH):
+                    raise RuntimeError(f"mom25d_amd: {LIB_PATH} is not built (make -C mom25d_amd/csrc); the HIP "
+                                       f"kernels are the only implementation")
+                h = C.CDLL(LIB_PATH)
+                h.m25x_last_error.restype = C.c_char_p
+                h.m25x_last_error.argtypes = []
+                for name, (args, res) in _SIGS.items():
+                    f = getattr(h, name)
+                    f.argtypes, f.restype = args, res
+                _lib = h
+    return _lib
+
+
+def check(rc, what=''):
+    """Raises the reference's exception types for the C ABI's codes."""
+    if rc == 0:
+        return
+    msg = lib().m25x_last_error().decode() or what
+    if rc == EINVAL:
+        raise ValueError(msg)
+    if rc == ENOMEM:
+        raise MemoryError(msg)
+    raise M25xError(msg)
+
+
+def call(name, *args):
+    check(getattr(lib(), name)(*args), name)
+
+
+def init(device_index):
+    """m25x_init once per device (checks that it is a gfx950)."""
+    if device_index not in _inited:
+        call('m25x_init', device_index)
+        _inited.add(device_index)

@@ -1,0 +1,671 @@
+// m25x_lu.hip - dense complex LU with partial pivoting on gfx950, and the triangular solves with its factor.
+//
+// A is column-major (the row-major Z of the reference seen as A = Z^T, the layout of scipy's lu_factor(Z.T)).
+// Blocked right-looking, panels of NB columns:
+//   panel   one cooperative launch per panel: each workgroup keeps a slab of the panel's rows in LDS; every column
+//           costs one grid barrier (pivot search, row interchange, scaling and the panel's rank-1 update in
+//           between, the next column's candidates published before the barrier); pivots by LAPACK's rule
+//           (largest |Re| + |Im|, the first on ties)
+//   laswp   the panel's interchanges on the columns left and right of it
+//   trsm    U12 = L11^-1 A12 (unit lower, forward substitution, one workgroup per 16 columns)
+//   gemm    A22 -= A21 U12 on the matrix cores: complex products as real ones through the real embedding
+//           (Re C = [Re A, Im A] [Re B; -Im B], Im C = [Re A, Im A] [Im B; Re B]), v_mfma_f64_16x16x4 for
+//           complex128 and v_mfma_f32_16x16x4 for complex64 (FP32 accumulation)
+// The solves (getrs) run one workgroup per 64-row block with ready flags between blocks (no launch per block).
+#include <algorithm>
+#include <mutex>
+#include "m25x_common.h"
+
+using namespace m25x;
+
+namespace {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// ---- complex traits ----------------------------------------------------------------------------------------
+template <typename T> struct CX;
+template <> struct CX<double2> {
+    typedef double R;
+    typedef d4v acc4;
+    __device__ static double2 make(double x, double y) { return make_double2(x, y); }
+    __device__ static acc4 mfma(double a, double b, acc4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    // the output row of lane group kk (lane / 16), register r (measured on gfx950: kk + 4 r for the f64 form)
+    __device__ static int out_row(int kk, int r) { return kk + 4 * r; }
+};
+template <> struct CX<float2> {
+    typedef float R;
+    typedef f4v acc4;
+    __device__ static float2 make(float x, float y) { return make_float2(x, y); }
+    __device__ static acc4 mfma(float a, float b, acc4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    // (4 kk + r for the f32 form)
+    __device__ static int out_row(int kk, int r) { return 4 * kk + r; }
+};
+
+template <typename T> __device__ inline T tadd(T a, T b) { return CX<T>::make(a.x + b.x, a.y + b.y); }
+template <typename T> __device__ inline T tsub(T a, T b) { return CX<T>::make(a.x - b.x, a.y - b.y); }
+template <typename T> __device__ inline T tmul(T a, T b)
+{
+    return CX<T>::make(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// reciprocal by Smith's algorithm
+template <typename T> __device__ inline T trecip(T b)
+{
+    typedef typename CX<T>::R R;
+    if (fabs((double)b.x) >= fabs((double)b.y)) {
+        R r = b.y / b.x, den = b.x + b.y * r;
+        return CX<T>::make((R)1 / den, -r / den);
+    }
+    R r = b.x / b.y, den = b.x * r + b.y;
+    return CX<T>::make(r / den, (R)-1 / den);
+}
+template <typename T> __device__ inline T tdiv(T a, T b)
+{
+    typedef typename CX<T>::R R;
+    if (fabs((double)b.x) >= fabs((double)b.y)) {
+        R r = b.y / b.x, den = b.x + b.y * r;
+        return CX<T>::make((a.x + a.y * r) / den, (a.y - a.x * r) / den);
+    }
+    R r = b.x / b.y, den = b.x * r + b.y;
+    return CX<T>::make((a.x * r + a.y) / den, (a.y * r - a.x) / den);
+}
+template <typename T> __device__ inline double cabs1(T a) { return fabs((double)a.x) + fabs((double)a.y); }
+
+// ---- grid barrier (all workgroups of a cooperative launch co-resident) -------------------------------------
+struct Bar {
+    unsigned count, gen;
+    int abort, pad;
+};
+
+// Sense by generation; release / acquire at agent scope orders the slot writes and reads across the XCDs.
+// A bounded wait: past the limit the barrier sets abort and every later barrier returns at once, so the
+// launch always drains (the host then reports the failure).
+__device__ inline void grid_sync(Bar *b, unsigned nblocks, unsigned &gen)
+{
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (!__hip_atomic_load(&b->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            const unsigned arrived = __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (arrived == nblocks - 1) {
+                __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                long spins = 0;
+                while (__hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > (1L << 26)) {
+                        __hip_atomic_store(&b->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+            }
+        }
+        gen += 1;
+    }
+    __syncthreads();
+}
+
+__global__ void k_zero_ints(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ---- panel --------------------------------------------------------------------------------------------------
+constexpr int NB = 128;          // panel width
+constexpr int PT = 256;          // panel threads
+
+template <typename T> struct Slot {
+    double val;
+    int row, pad;
+    T v[NB];
+};
+
+// argmax of |Re| + |Im| over (val, row) pairs: larger value, then smaller row
+__device__ inline void better(double &v, int &r, double v2, int r2)
+{
+    if (v2 > v || (v2 == v && r2 < r && r2 >= 0) || (r < 0 && r2 >= 0)) { v = v2; r = r2; }
+}
+
+__device__ inline void block_argmax(double &v, int &r, double *sv, int *sr)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const double v2 = __shfl_xor(v, o);
+        const int r2 = __shfl_xor(r, o);
+        better(v, r, v2, r2);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { sv[wv] = v; sr[wv] = r; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < PT / 64; ++k) better(sv[0], sr[0], sv[k], sr[k]);
+    }
+    __syncthreads();
+    v = sv[0]; r = sr[0];
+    __syncthreads();
+}
+
+// Slab of workgroup g: global rows [r0, r0 + rn), panel columns [j0, j0 + jb), column-major in LDS (S[c * R + r]).
+template <typename T>
+__global__ __launch_bounds__(PT) void k_panel(T *A, long lda, long n, long j0, int jb, int R, int *ipiv, int *info,
+                                              Slot<T> *slots, T *diag, Bar *bar)
+{
+    extern __shared__ char smem_raw[];
+    T *S = (T *)smem_raw;
+    __shared__ T urow[NB], drow[NB];
+    __shared__ double sv[PT / 64];
+    __shared__ int sr[PT / 64];
+    __shared__ int piv_row;
+    const unsigned G = gridDim.x;
+    const int g = blockIdx.x, tid = threadIdx.x;
+    const long r0 = j0 + (long)g * R;
+    const int rn = (int)std::max(0L, std::min((long)R, n - r0));
+    unsigned gen = 0;
+    for (int idx = tid; idx < rn * jb; idx += PT) {
+        const int c = idx / rn, r = idx % rn;
+        S[c * R + r] = A[(j0 + c) * lda + r0 + r];
+    }
+    __syncthreads();
+    // candidates of column k: rows >= j0 + k of my slab; publish (val, row, row values) in slots[buf][g], and the
+    // diagonal row j0 + k in diag[buf] when it is mine
+    auto publish = [&](int k, int buf) {
+        double v = -1.0;
+        int rr = -1;
+        const long first = j0 + k;
+        for (int r = tid; r < rn; r += PT)
+            if (r0 + r >= first) better(v, rr, cabs1(S[k * R + r]), (int)(r0 + r));
+        block_argmax(v, rr, sv, sr);
+        Slot<T> *sl = slots + (size_t)buf * G + g;
+        if (tid == 0) { sl->val = v; sl->row = rr; }
+        if (rr >= 0)
+            for (int c = tid; c < jb; c += PT) sl->v[c] = S[c * R + (rr - r0)];
+        if (first >= r0 && first < r0 + rn)
+            for (int c = tid; c < jb; c += PT) diag[(size_t)buf * NB + c] = S[c * R + (first - r0)];
+    };
+    publish(0, 0);
+    grid_sync(bar, G, gen);
+    for (int k = 0; k < jb; ++k) {
+        const int buf = k & 1;
+        const long dr = j0 + k;
+        // the pivot: the best of the published candidates (every workgroup reads the same ones)
+        if (tid < 64) {
+            double v = -1.0;
+            int rr = -1;
+            for (unsigned q = tid; q < G; q += 64) {
+                const Slot<T> *sl = slots + (size_t)buf * G + q;
+                better(v, rr, sl->val, sl->row);
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                const double v2 = __shfl_xor(v, o);
+                const int r2 = __shfl_xor(rr, o);
+                better(v, rr, v2, r2);
+            }
+            if (tid == 0) piv_row = rr < 0 ? (int)dr : rr;
+        }
+        __syncthreads();
+        const int p = piv_row;
+        const int gp = (int)((p - j0) / R);
+        for (int c = tid; c < jb; c += PT) {
+            urow[c] = slots[(size_t)buf * G + gp].v[c];
+            drow[c] = diag[(size_t)buf * NB + c];
+        }
+        if (g == 0 && tid == 0) ipiv[dr] = p;
+        __syncthreads();
+        if (p != dr) {
+            if (p >= r0 && p < r0 + rn)
+                for (int c = tid; c < jb; c += PT) S[c * R + (p - r0)] = drow[c];
+            if (dr >= r0 && dr < r0 + rn)
+                for (int c = tid; c < jb; c += PT) S[c * R + (dr - r0)] = urow[c];
+        }
+        __syncthreads();
+        const T pv = urow[k];
+        const bool zero = pv.x == 0 && pv.y == 0;
+        if (zero && g == 0 && tid == 0 && info && *info == 0) *info = (int)dr + 1;
+        const T inv = zero ? CX<T>::make(0, 0) : trecip(pv);
+        // multipliers, then the rank-1 update of the rest of the slab's panel columns
+        const int rbeg = (int)std::max(0L, std::min((long)rn, dr + 1 - r0));
+        if (!zero)
+            for (int r = rbeg + tid; r < rn; r += PT) S[k * R + r] = tmul(S[k * R + r], inv);
+        __syncthreads();
+        const int nr = rn - rbeg, ncol = jb - k - 1;
+        for (int idx = tid; idx < nr * ncol; idx += PT) {
+            const int c = k + 1 + idx / nr, r = rbeg + idx % nr;
+            S[c * R + r] = tsub(S[c * R + r], tmul(S[k * R + r], urow[c]));
+        }
+        __syncthreads();
+        if (k + 1 < jb) {
+            publish(k + 1, buf ^ 1);
+            grid_sync(bar, G, gen);
+        }
+    }
+    for (int idx = tid; idx < rn * jb; idx += PT) {
+        const int c = idx / rn, r = idx % rn;
+        A[(j0 + c) * lda + r0 + r] = S[c * R + r];
+    }
+}
+
+// the panel's interchanges on columns [0, j0) and [j0 + jb, n): one thread a column, in order
+template <typename T>
+__global__ void k_laswp(T *A, long lda, long n, long j0, int jb, const int *ipiv)
+{
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long nleft = j0, nright = n - j0 - jb;
+    if (t >= nleft + nright) return;
+    const long c = t < nleft ? t : j0 + jb + (t - nleft);
+    T *col = A + c * lda;
+    for (int k = 0; k < jb; ++k) {
+        const long r = j0 + k, p = ipiv[r];
+        if (p != r) {
+            const T a = col[r];
+            col[r] = col[p];
+            col[p] = a;
+        }
+    }
+}
+
+// U12 = L11^-1 A12: rows [j0, j0 + jb), columns [c0, n); 32 columns a workgroup, the block in LDS
+constexpr int TC = 16;
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm(T *A, long lda, long n, long j0, int jb, long c0)
+{
+    __shared__ T X[NB][TC + 1];
+    const long cb = c0 + (long)blockIdx.x * TC;
+    const int nc = (int)std::min((long)TC, n - cb);
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < jb * TC; idx += 256) {
+        const int c = idx / jb, r = idx % jb;
+        if (c < nc) X[r][c] = A[(cb + c) * lda + j0 + r];
+    }
+    __syncthreads();
+    const T *L = A + j0 * lda + j0;          // L11[i][k] = L[k * lda + i]
+    for (int k = 0; k < jb - 1; ++k) {
+        const int nr = jb - k - 1;
+        for (int idx = tid; idx < nr * TC; idx += 256) {
+            const int c = idx % TC, i = k + 1 + idx / TC;
+            if (c < nc) X[i][c] = tsub(X[i][c], tmul(L[(long)k * lda + i], X[k][c]));
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < jb * TC; idx += 256) {
+        const int c = idx / jb, r = idx % jb;
+        if (c < nc) A[(cb + c) * lda + j0 + r] = X[r][c];
+    }
+}
+
+// ---- C -= A B on the matrix cores --------------------------------------------------------------------------
+// C (M x N), A (M x K), B (K x N), column-major, complex. Workgroup tile 64 x 64, four waves of 32 x 32, each
+// 2 x 2 blocks of 16 x 16 with a real and an imaginary accumulator. The MFMA's first operand is B (its rows are
+// C's columns), its second A, so a lane's outputs run down C's columns (coalesced stores).
+constexpr int GM = 64, GN = 64, GK = 8;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_sub(T *C, long ldc, const T *A, long lda, const T *B, long ldb, long M,
+                                                  long N, long K)
+{
+    typedef typename CX<T>::R R;
+    typedef typename CX<T>::acc4 acc4;
+    __shared__ T As[GK][GM];         // As[p][i] = A[i][p]
+    __shared__ T Bs[GK][GN + 1];     // Bs[p][j] = B[p][j]
+    const long m0 = (long)blockIdx.x * GM, n0 = (long)blockIdx.y * GN;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;     // wave tile origin in the workgroup tile
+    const int li = lane & 15, kk = lane >> 4, part = kk & 1, ph = kk >> 1;
+    acc4 cr[2][2], ci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) { cr[a][b][r] = 0; ci[a][b][r] = 0; }
+    const T zero = CX<T>::make(0, 0);
+    for (long k0 = 0; k0 < K; k0 += GK) {
+        // stage A (GM x GK) and B (GK x GN); zero outside
+        for (int idx = tid; idx < GM * GK; idx += 256) {
+            const int i = idx % GM, p = idx / GM;
+            const long gi = m0 + i, gp = k0 + p;
+            As[p][i] = (gi < M && gp < K) ? A[gp * lda + gi] : zero;
+        }
+        for (int idx = tid; idx < GK * GN; idx += 256) {
+            const int p = idx % GK, j = idx / GK;
+            const long gp = k0 + p, gj = n0 + j;
+            Bs[p][j] = (gp < K && gj < N) ? B[gj * ldb + gp] : zero;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < GK / 2; ++ks) {
+            const int p = 2 * ks + ph;
+            R av[2], br[2], bi[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const T x = As[p][wm + a * 16 + li];
+                av[a] = part ? x.y : x.x;
+            }
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const T y = Bs[p][wn + b * 16 + li];
+                br[b] = part ? -y.y : y.x;
+                bi[b] = part ? y.x : y.y;
+            }
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int a = 0; a < 2; ++a) {
+                    cr[b][a] = CX<T>::mfma(br[b], av[a], cr[b][a]);
+                    ci[b][a] = CX<T>::mfma(bi[b], av[a], ci[b][a]);
+                }
+        }
+        __syncthreads();
+    }
+    // lane (li, kk), register r: C row m0 + wm + a*16 + li, column n0 + wn + b*16 + out_row(kk, r)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long gi = m0 + wm + a * 16 + li, gj = n0 + wn + b * 16 + CX<T>::out_row(kk, r);
+                if (gi < M && gj < N) {
+                    T *c = C + gj * ldc + gi;
+                    const T v = *c;
+                    *c = CX<T>::make(v.x - cr[b][a][r], v.y - ci[b][a][r]);
+                }
+            }
+}
+
+// ---- the interchanges as a permutation: perm = the swaps of ipiv applied in order to 0..n-1 ------------------
+__global__ void k_perm(const int *ipiv, long n, int *perm)
+{
+    // one workgroup: identity in parallel, then the swaps in order by one thread, ipiv read 8 at a time (the
+    // loads are independent of the swaps; most pivots are on the diagonal)
+    for (long i = threadIdx.x; i < n; i += blockDim.x) perm[i] = (int)i;
+    __threadfence_block();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (long i0 = 0; i0 < n; i0 += 8) {
+            int pv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) pv[u] = i0 + u < n ? ipiv[i0 + u] : (int)(i0 + u);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const long i = i0 + u;
+                const int p = pv[u];
+                if (i < n && p != i) {
+                    const int t = perm[i];
+                    perm[i] = perm[p];
+                    perm[p] = t;
+                }
+            }
+        }
+    }
+}
+
+// ---- triangular solves, one workgroup a 64-row block, ready flags between blocks ---------------------------
+// B: n x nrhs row-major (ldb). The blocks of the solution are published in order; block b adds the
+// contributions of blocks 0..b-1 as their flags come up, then solves its diagonal block.
+constexpr int SB = 64;
+constexpr int MAXRHS = 8;
+
+// lower: U^T (trans, forward) with U[k][i] = LU[i * lda + k], non-unit; or L (no trans, forward), unit.
+// upper: L^T (trans, backward) with L[k][i] = LU[i * lda + k], unit; or U (no trans, backward), non-unit.
+// kind 0: U^T y = b; 1: L^T z = y; 2: L y = b; 3: U x = y.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv(const T *LU, long lda, long n, T *B, long ldb, int nrhs, int kind,
+                                              int *flags, int *abort)
+{
+    __shared__ T D[SB][SB + 1];          // the diagonal block, D[i][k] = the operator's entry (row i, column k)
+    __shared__ T acc[SB][MAXRHS];
+    __shared__ T part[2][SB][MAXRHS];
+    const int nb = (int)((n + SB - 1) / SB);
+    const bool fwd = kind == 0 || kind == 2;
+    const int b = fwd ? blockIdx.x : nb - 1 - blockIdx.x;     // the order of publication is blockIdx order
+    const long i0 = (long)b * SB;
+    const int bn = (int)std::min((long)SB, n - i0);
+    const int tid = threadIdx.x;
+    // operator entry (row i, column k), global indices
+    auto op = [&](long i, long k) -> T {
+        switch (kind) {
+        case 0: return LU[i * lda + k];      // U^T[i][k] = U[k][i]
+        case 1: return LU[i * lda + k];      // L^T[i][k] = L[k][i]
+        case 2: return LU[k * lda + i];      // L[i][k]
+        default: return LU[k * lda + i];     // U[i][k]
+        }
+    };
+    for (int idx = tid; idx < SB * SB; idx += 256) {
+        const int i = idx / SB, k = idx % SB;
+        D[i][k] = (i < bn && k < bn) ? op(i0 + i, i0 + k) : CX<T>::make(0, 0);
+    }
+    for (int idx = tid; idx < SB * MAXRHS; idx += 256) {
+        const int i = idx / MAXRHS, r = idx % MAXRHS;
+        acc[i][r] = (i < bn && r < nrhs) ? B[(i0 + i) * ldb + r] : CX<T>::make(0, 0);
+    }
+    __syncthreads();
+    // contributions of the blocks before this one, in publication order
+    const int row = tid & 63, ks = (tid >> 6) & 1;   // 2 k-slices of 32 (threads 128-255 idle here)
+    for (int q = 0; q < (int)blockIdx.x; ++q) {
+        const int a = fwd ? q : nb - 1 - q;
+        if (tid == 0) {
+            long spins = 0;
+            while (__hip_atomic_load(flags + a, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > (1L << 26)) { __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
+            }
+        }
+        __syncthreads();
+        const long k0 = (long)a * SB;
+        const int kn = (int)std::min((long)SB, n - k0);
+        T s[MAXRHS];
+#pragma unroll
+        for (int r = 0; r < MAXRHS; ++r) s[r] = CX<T>::make(0, 0);
+        if (row < bn && tid < 128) {
+            for (int kx = ks * 32; kx < std::min(kn, ks * 32 + 32); ++kx) {
+                const T m = op(i0 + row, k0 + kx);
+                const T *y = B + (k0 + kx) * ldb;
+                for (int r = 0; r < nrhs; ++r) s[r] = tadd(s[r], tmul(m, y[r]));
+            }
+        }
+        if (tid < 128)
+            for (int r = 0; r < nrhs; ++r) part[ks][row][r] = s[r];
+        __syncthreads();
+        for (int idx = tid; idx < SB * nrhs; idx += 256) {
+            const int i = idx / nrhs, r = idx % nrhs;
+            acc[i][r] = tsub(acc[i][r], tadd(part[0][i][r], part[1][i][r]));
+        }
+        __syncthreads();
+    }
+    // the diagonal block: column-oriented substitution (forward or backward)
+    const bool unit = kind == 1 || kind == 2;
+    for (int st = 0; st < bn; ++st) {
+        const int k = fwd ? st : bn - 1 - st;
+        if (!unit) {
+            if (tid < nrhs) acc[k][tid] = tdiv(acc[k][tid], D[k][k]);
+            __syncthreads();
+        }
+        for (int idx = tid; idx < SB * nrhs; idx += 256) {
+            const int i = idx / nrhs, r = idx % nrhs;
+            const bool after = fwd ? (i > k && i < bn) : (i < k);
+            if (after) acc[i][r] = tsub(acc[i][r], tmul(D[i][k], acc[k][r]));
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < bn * nrhs; idx += 256) {
+        const int i = idx / nrhs, r = idx % nrhs;
+        B[(i0 + i) * ldb + r] = acc[i][r];
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(flags + b, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// out[perm[i]] = in[i] (scatter, trans) or out[i] = in[perm[i]] (gather)
+template <typename T>
+__global__ void k_permute(const T *in, T *out, long n, int nrhs, long ld, const int *perm, int scatter)
+{
+    const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (idx >= n * nrhs) return;
+    const long i = idx / nrhs;
+    const int r = (int)(idx % nrhs);
+    if (scatter) out[(long)perm[i] * ld + r] = in[i * ld + r];
+    else out[i * ld + r] = in[(long)perm[i] * ld + r];
+}
+
+inline unsigned blocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename T> size_t work_bytes(long n)
+{
+    const int cus = 256;
+    return 2 * (size_t)cus * sizeof(Slot<T>) + 2 * NB * sizeof(T) + 256;
+}
+
+template <typename T> int max_rows()
+{
+    // LDS for the slab: at most 128 KB
+    return (int)((128 * 1024) / (NB * sizeof(T)));
+}
+
+template <typename T>
+int getrf_t(T *A, long n, long lda, int *ipiv, int *info, void *work, size_t wbytes, hipStream_t s)
+{
+    if (wbytes < work_bytes<T>(n)) return fail(M25X_EINVAL, "getrf: workspace of %zu bytes, %zu needed", wbytes, work_bytes<T>(n));
+    const int cus = std::min(num_cus(), 256);
+    char *w = (char *)work;
+    Slot<T> *slots = (Slot<T> *)w;
+    T *diag = (T *)(w + 2 * (size_t)256 * sizeof(Slot<T>));
+    Bar *bar = (Bar *)(w + 2 * (size_t)256 * sizeof(Slot<T>) + 2 * NB * sizeof(T));
+    const int Rmax = max_rows<T>();
+    static bool attr_done[2] = {false, false};
+    const size_t smem_max = (size_t)Rmax * NB * sizeof(T);
+    {
+        static std::mutex mu;
+        std::lock_guard<std::mutex> lk(mu);
+        if (!attr_done[sizeof(T) == 8]) {
+            M25X_HIP(hipFuncSetAttribute((const void *)k_panel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem_max));
+            attr_done[sizeof(T) == 8] = true;
+        }
+    }
+    if (info) {
+        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, info, 1);
+        if (int rc = check_launch("getrf")) return rc;
+    }
+    for (long j0 = 0; j0 < n; j0 += NB) {
+        const int jb = (int)std::min((long)NB, n - j0);
+        const long m = n - j0;
+        int G = (int)std::min((long)cus, (m + Rmax - 1) / Rmax);
+        G = std::max(G, 1);
+        int R = (int)((m + G - 1) / G);
+        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, (int *)bar, 4);
+        if (int rc = check_launch("getrf")) return rc;
+        const size_t smem = (size_t)R * jb * sizeof(T);
+        void *args[] = {(void *)&A, (void *)&lda, (void *)&n, (void *)&j0, (void *)&jb, (void *)&R, (void *)&ipiv,
+                        (void *)&info, (void *)&slots, (void *)&diag, (void *)&bar};
+        M25X_HIP(hipLaunchCooperativeKernel((const void *)k_panel<T>, dim3(G), dim3(PT), args, (unsigned)smem, s));
+        const long nsw = n - jb;
+        if (nsw > 0) {
+            hipLaunchKernelGGL(k_laswp<T>, dim3(blocks(nsw, 256)), dim3(256), 0, s, A, lda, n, j0, jb, ipiv);
+            if (int rc = check_launch("getrf/laswp")) return rc;
+        }
+        const long c0 = j0 + jb;
+        if (c0 < n) {
+            hipLaunchKernelGGL(k_trsm<T>, dim3(blocks(n - c0, TC)), dim3(256), 0, s, A, lda, n, j0, jb, c0);
+            if (int rc = check_launch("getrf/trsm")) return rc;
+            const long M = n - c0;
+            dim3 grid(blocks(M, GM), blocks(M, GN));
+            hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, s, A + c0 * lda + c0, lda, A + j0 * lda + c0, lda,
+                               A + c0 * lda + j0, lda, M, M, (long)jb);
+            if (int rc = check_launch("getrf/gemm")) return rc;
+        }
+    }
+    return M25X_OK;
+}
+
+template <typename T> size_t getrs_work(long n, long ldb)
+{
+    const long nb = (n + SB - 1) / SB;
+    return (size_t)n * ldb * sizeof(T) + sizeof(int) * ((size_t)n + 2 * nb + 64);
+}
+
+template <typename T>
+int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, T *B, long ldb, int nrhs, int trans,
+            void *work, size_t wbytes, hipStream_t s)
+{
+    if (nrhs < 1 || nrhs > MAXRHS) return fail(M25X_EINVAL, "getrs: 1 to %d right-hand sides, not %d", MAXRHS, nrhs);
+    if (ldb < nrhs) return fail(M25X_EINVAL, "getrs: ldb < nrhs");
+    if (!work || wbytes < getrs_work<T>(n, ldb))
+        return fail(M25X_EINVAL, "getrs: workspace of %zu bytes, %zu needed", wbytes, getrs_work<T>(n, ldb));
+    const int nb = (int)((n + SB - 1) / SB);
+    const size_t tmp_bytes = (size_t)n * ldb * sizeof(T);
+    T *tmp = (T *)work;
+    int *perm = (int *)((char *)work + tmp_bytes);
+    int *flags = perm + n;
+    int *abort = flags + 2 * nb;
+    hipLaunchKernelGGL(k_zero_ints, dim3(blocks(2 * nb + 1, 256)), dim3(256), 0, s, flags, 2 * nb + 1);
+    if (!perm_in) {
+        hipLaunchKernelGGL(k_perm, dim3(1), dim3(256), 0, s, ipiv, n, perm);
+        perm_in = perm;
+    }
+    if (int rc = check_launch("getrs")) return rc;
+    if (trans) {
+        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, B, ldb, nrhs, 0, flags, abort);
+        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, B, ldb, nrhs, 1, flags + nb, abort);
+        hipLaunchKernelGGL(k_permute<T>, dim3(blocks(n * nrhs, 256)), dim3(256), 0, s, (const T *)B, tmp, n, nrhs, ldb,
+                           perm_in, 1);
+    } else {
+        hipLaunchKernelGGL(k_permute<T>, dim3(blocks(n * nrhs, 256)), dim3(256), 0, s, (const T *)B, tmp, n, nrhs, ldb,
+                           perm_in, 0);
+        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, tmp, ldb, nrhs, 2, flags, abort);
+        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, tmp, ldb, nrhs, 3, flags + nb, abort);
+    }
+    if (int rc = check_launch("getrs")) return rc;
+    M25X_HIP(hipMemcpyAsync(B, tmp, tmp_bytes, hipMemcpyDeviceToDevice, s));
+    return M25X_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t m25x_getrf_work_bytes(int dtype, long n)
+{
+    return dtype == M25X_C64 ? work_bytes<float2>(n) : work_bytes<double2>(n);
+}
+
+int m25x_getrf(int dtype, void *A, long n, long lda, int *ipiv, int *info, void *work, size_t wbytes, void *stream)
+{
+    if (!A || !ipiv || !work || n < 0 || lda < n) return fail(M25X_EINVAL, "getrf: bad arguments");
+    if (n == 0) return M25X_OK;
+    if (n > (1L << 30)) return fail(M25X_EINVAL, "getrf: n too large");
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == M25X_C128) return getrf_t<double2>((double2 *)A, n, lda, ipiv, info, work, wbytes, s);
+    if (dtype == M25X_C64) return getrf_t<float2>((float2 *)A, n, lda, ipiv, info, work, wbytes, s);
+    return fail(M25X_EINVAL, "getrf: dtype %d", dtype);
+}
+
+size_t m25x_getrs_work_bytes(int dtype, long n, long ldb)
+{
+    return dtype == M25X_C64 ? getrs_work<float2>(n, ldb) : getrs_work<double2>(n, ldb);
+}
+
+int m25x_ipiv_perm(const int *ipiv, long n, int *perm, void *stream)
+{
+    if (!ipiv || !perm || n < 0) return fail(M25X_EINVAL, "ipiv_perm: bad arguments");
+    if (n == 0) return M25X_OK;
+    hipLaunchKernelGGL(k_perm, dim3(1), dim3(256), 0, (hipStream_t)stream, ipiv, n, perm);
+    return check_launch("ipiv_perm");
+}
+
+int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B, long ldb,
+               int nrhs, int trans, void *work, size_t work_bytes, void *stream)
+{
+    if (!LU || !ipiv || !B || n < 0 || lda < n) return fail(M25X_EINVAL, "getrs: bad arguments");
+    if (n == 0) return M25X_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == M25X_C128)
+        return getrs_t<double2>((const double2 *)LU, n, lda, ipiv, perm, (double2 *)B, ldb, nrhs, trans, work,
+                                work_bytes, s);
+    if (dtype == M25X_C64)
+        return getrs_t<float2>((const float2 *)LU, n, lda, ipiv, perm, (float2 *)B, ldb, nrhs, trans, work,
+                               work_bytes, s);
+    return fail(M25X_EINVAL, "getrs: dtype %d", dtype);
+}
+
+}  // extern "C"

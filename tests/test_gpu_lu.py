@@ -1,0 +1,98 @@
+"""The GPU LU and its solves against LAPACK (SciPy's lu_factor / lu_solve), the reference's factorisation
+(src/mom25d/fast.py:318-358) and its GPU-backend test (tests/test_gpu.py:210-235 of the reference): the same
+pivots, the factor in lu_factor's layout, and the solves of both transposes.
+
+Tolerances: complex128 - backward error 1e-13 and the factor within 1e-12 of LAPACK's; complex64 - the
+reference's own bounds, |lu - ref| < 1e-4 max|ref| and solves within 1e-4 (test_gpu.py:231-235).
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import torch
+
+from tests.cases import seeded_matrix
+
+pytestmark = pytest.mark.gpu
+
+
+def _factor(dv, Z):
+    Zd = torch.from_numpy(np.ascontiguousarray(Z)).to(dv.device())
+    F = dv.getrf(Zd)
+    torch.cuda.synchronize()
+    return F
+
+
+@pytest.mark.parametrize('n', [1, 7, 64, 129, 300, 517])
+def test_getrf_c128_matches_lapack(gpu, n):
+    dv = gpu
+    Z = seeded_matrix(n)
+    F = _factor(dv, Z)
+    lu_ref, piv_ref = sla.lu_factor(Z.T)                   # the reference's factor of A = Z^T
+    lu = F.LU.cpu().numpy().T                              # our row-major buffer is A column-major
+    piv = F.ipiv.cpu().numpy()
+    assert np.array_equal(piv, piv_ref)
+    assert np.abs(lu - lu_ref).max() <= 1e-12 * np.abs(lu_ref).max()
+    assert int(F.info.item()) == 0
+    rng = np.random.default_rng(n)
+    R = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    X = dv.getrs(F, torch.from_numpy(R).to(dv.device()), trans=1).cpu().numpy()
+    Xr = sla.lu_solve((lu_ref, piv_ref), R, trans=1)      # Z X = R
+    assert np.abs(X - Xr).max() <= 1e-11 * np.abs(Xr).max()
+    assert np.abs(Z @ X - R).max() <= 1e-12 * np.abs(R).max() * np.abs(Z).max() * n ** 0.5
+    X0 = dv.getrs(F, torch.from_numpy(R).to(dv.device()), trans=0).cpu().numpy()
+    assert np.abs(Z.T @ X0 - R).max() <= 1e-12 * np.abs(R).max() * np.abs(Z).max() * n ** 0.5
+    x1 = dv.getrs(F, torch.from_numpy(R[:, 0].copy()).to(dv.device()), trans=1).cpu().numpy()
+    assert np.abs(x1 - Xr[:, 0]).max() <= 1e-11 * np.abs(Xr).max()
+
+
+@pytest.mark.parametrize('n', [300, 1000])
+def test_getrf_c64_matches_lapack(gpu, n):
+    """complex64: the reference's bounds for its GPU LU (tests/test_gpu.py:219-235)."""
+    dv = gpu
+    Z = seeded_matrix(n, dtype=np.complex64)
+    F = _factor(dv, Z)
+    lu_ref, piv_ref = sla.lu_factor(Z.T.astype(complex))
+    assert np.array_equal(F.ipiv.cpu().numpy(), piv_ref)
+    lu = F.LU.cpu().numpy().T
+    assert np.abs(lu - lu_ref).max() < 1e-4 * np.abs(lu_ref).max()
+    rng = np.random.default_rng(3)
+    R = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    x = np.linalg.solve(Z.astype(complex), R)
+    X = dv.getrs(F, torch.from_numpy(R).to(dv.device()), trans=1).cpu().numpy()
+    assert X.dtype == np.complex128
+    assert np.abs(X - x).max() < 1e-4 * np.abs(x).max()
+
+
+def test_getrf_selfcheck_matrix(gpu):
+    """tools/gpu_selfcheck.py's matrix (seed 1, (randn + i randn) / sqrt(n) + 3 I): residual < 1e-4 (complex64)
+    and ~1e-14 (complex128)."""
+    dv = gpu
+    n = 2048
+    rng = np.random.default_rng(1)
+    A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))).astype(np.complex64) / np.sqrt(n)
+    A[np.diag_indices(n)] += 3
+    b = rng.standard_normal((n, 2)).astype(complex)
+    for dt, tol in ((np.complex64, 1e-4), (np.complex128, 1e-12)):
+        F = _factor(dv, A.astype(dt))
+        x = dv.getrs(F, torch.from_numpy(b).to(dv.device()), trans=1).cpu().numpy()
+        res = np.linalg.norm(A.astype(complex) @ x - b) / np.linalg.norm(b)
+        assert res < tol, (dt, res)
+
+
+def test_zgemm_nk(gpu):
+    dv = gpu
+    rng = np.random.default_rng(2)
+    for n, k in ((1, 1), (300, 3), (1000, 8), (777, 11)):
+        Z = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        X = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
+        Y = dv.zgemm_nk(torch.from_numpy(Z).to(dv.device()), torch.from_numpy(X).to(dv.device())).cpu().numpy()
+        assert np.abs(Y - Z @ X).max() <= 1e-13 * np.abs(Z).max() * np.abs(X).max() * n
+
+
+def test_singular_pivot_reports_info(gpu):
+    dv = gpu
+    Z = seeded_matrix(100)
+    Z[:, 5] = 0                                            # column 5 of Z = row 5 of A: zero from the start
+    Z[5, :] = 0
+    F = _factor(dv, Z)
+    assert int(F.info.item()) == 6                        # LAPACK's info: 1 + the first zero pivot
