@@ -1,0 +1,191 @@
+"""The direct solver of one frequency on the GPU (src/mom25d/fast.py: FastOperator.setup / solve_direct /
+_solve_kept / _refine / _reusable, 597-626, 704-850, 974-1084).
+
+Per frequency, everything on the device:
+  tables   the Green's tables of every interface pair (multilayer.green_tables)
+  fill     Z (N x N complex128): far field, near-field correction (pairs and their analytic moments resident
+           across frequencies), via rows / columns and block, lumped elements
+  solve    'double': Z equilibrated (D Z D, d = 1/sqrt|Z_ii|), complex128 LU, solves
+           'mixed' : the complex64 copy of the equilibrated Z factored (FP32 matrix cores), GMRES refinement
+                     against the FP64 Z (restart 30, stall (4, 0.1) for a fresh factor, converged at 100 x 1e-6);
+                     a kept factor of a nearby frequency tried first (its reach learnt); where the complex64
+                     factor does not converge, complex128 LU, and double precision up to MIXED_RETRY x f
+The currents come back to the host (N x ports); everything else stays on the GPU.
+"""
+import time
+
+import numpy as np
+import torch
+
+from . import device as dv
+from .linalg import gmres_batched
+
+HYBRID_RETRY = 1.25
+MIXED_RETRY = HYBRID_RETRY
+MIXED_MAX_ITER = 30
+FRESH_STALL = (4, 0.1)
+REUSE_ITER = (8, 60)
+REUSE_KEEP = 4
+
+
+class FastOperator:
+    """Builds Z on the GPU for a Simulation and solves it."""
+
+    def __init__(self, sim, near_fac=None):
+        self.sim = sim
+        self.near_fac = near_fac or sim.near_fac
+        self.mixed_tol = 1e-6
+        self._mixed_failed = 0.0
+        self._kept = []                       # [(f, Factor, dsc (device), seconds a fresh one took)]
+        self._t_iter = None
+        self._reach = np.log(2.0)
+        self.iters = []
+        self.mesh = dv.DevMesh(sim.g, sim.q)
+        with sim.log.timer('near pairs', sync=True):
+            z = sim.st.z[sim.tri_layer]
+            P, Q = dv.near_pairs(sim.rwg.cent, sim.tri_size, z, self.near_fac)
+            self.plan = dv.NearPlan(self.mesh, P, Q)
+        sim.log.note(f"fast solver: {self.plan.n} near triangle pairs ({self.plan.n / max(1, len(sim.tri_size)):.0f} "
+                     f"per triangle), {self.plan.nentry} near entries")
+        F = sim.F
+        self._fan = None
+        if len(F):
+            dev = dv.device()
+            pos = -np.ones(len(sim.tri_size), np.int64)
+            pos[F] = np.arange(len(F))
+            Pn, Qn = self.plan.P.long().cpu().numpy(), self.plan.Q.long().cpu().numpy()
+            sel = np.flatnonzero(pos[Qn] >= 0)
+            self._fan = (torch.from_numpy(sel).to(dev), torch.from_numpy(Pn[sel]).to(dev),
+                         torch.from_numpy(pos[Qn[sel]]).to(dev))
+
+    # ---- set-up and fill ------------------------------------------------------------------------------------
+    def setup(self, f):
+        self.f, self.w = f, 2 * np.pi * f
+        self.sim.green_tables(f)
+        self.tabs = self.sim.dev_tables
+
+    def fill(self, f):
+        """Z (device, complex128, N x N) at f, not equilibrated."""
+        sim, w = self.sim, 2 * np.pi * f
+        Ne, Nv = sim.Ne, sim.Nv
+        N = Ne + Nv
+        self.setup(f)
+        dev = dv.device()
+        with sim.log.timer('direct fill', f"N={N}, complex128, GPU"):
+            Z = torch.empty((N, N), dtype=dv.c128, device=dev)
+            dv.fill_far(Z, self.mesh, self.tabs, w)
+        with sim.log.timer('near-field correction'):
+            M, qc = dv.near_moments(self.mesh, self.tabs, self.plan, w)
+            dv.near_scatter(Z, self.mesh, self.plan, M, w)
+        if Nv:
+            with sim.log.timer('via kernels', f"{Nv} chains"):
+                QF = dv.fan_q(self.mesh, self.tabs, sim.F)
+                if self._fan is not None:
+                    sel, rows, cols = self._fan
+                    QF[rows, cols] += qc[sel]
+                B, C = sim._via_blocks(f, QF)
+                Z[:Ne, Ne:] = B
+                Z[Ne:, :Ne] = B.T
+                Z[Ne:, Ne:] = C
+        lum = sim.lumped_entries(f)
+        if lum is not None:
+            r, c, v = (torch.from_numpy(x).to(dev) for x in lum)
+            Z.index_put_((r, c), v.to(dv.c128), accumulate=True)
+        return Z
+
+    # ---- solve ---------------------------------------------------------------------------------------------
+    def _refine(self, A, B, M, restart, maxiter, stall=None):
+        """GMRES to mixed_tol: (Y, converged, largest relative residual) (fast.py:658-663)."""
+        Y, self.iters = gmres_batched(A, B, M=M, rtol=self.mixed_tol, restart=restart, maxiter=maxiter, stall=stall)
+        rel = (torch.linalg.vector_norm(B - A(Y), dim=0) /
+               torch.clamp(torch.linalg.vector_norm(B, dim=0), min=1e-300)).cpu().numpy()
+        ok = bool(np.isfinite(rel).all() and (rel <= 100 * self.mixed_tol).all())
+        return Y, ok, float(np.max(rel)) if len(rel) else 0.0
+
+    def _reusable(self, f):
+        if self.sim.reuse == 'off' or not self._kept:
+            return None
+        k = min(self._kept, key=lambda k: abs(np.log(f / k[0])))
+        return k if abs(np.log(f / k[0])) <= self._reach else None
+
+    def _reach_learn(self, d, ok, iters, cap):
+        if ok and max(iters) <= cap // 2:
+            self._reach = max(self._reach, 1.25 * d)
+        elif not ok:
+            self._reach = min(self._reach, 0.8 * d)
+
+    def _drop_kept(self):
+        lf = np.log([k[0] for k in self._kept])
+        gap = [min((abs(lf[i] - y) for j, y in enumerate(lf) if j != i), default=0.0) for i in range(len(lf))]
+        del self._kept[int(np.argmin(gap))]
+
+    def solve_direct(self, f, E, precision='double'):
+        """Currents (host N x k complex128) for excitation columns E at f (fast.py:704-850)."""
+        sim = self.sim
+        if precision == 'mixed' and f <= MIXED_RETRY * self._mixed_failed:
+            precision = 'double'
+        t_fresh = time.time()
+        Z = self.fill(f)
+        dev = Z.device
+        Ed = torch.from_numpy(np.asarray(E, complex)).to(dev)
+        dsc = dv.diag_scales(Z)
+        B = dsc[:, None] * Ed
+        if precision != 'mixed':
+            with sim.log.timer('LU factor', f"N={Z.shape[0]}, complex128, GPU"):
+                dv.scale_c64(Z, dsc)
+                F = dv.getrf(Z)
+            self._check_info(F)
+            with sim.log.timer('LU solve'):
+                X = dsc[:, None] * dv.getrs(F, B, trans=1)
+            return X.cpu().numpy()
+        return self._solve_kept(f, Z, dsc, B, t_fresh).cpu().numpy()
+
+    def _check_info(self, F):
+        info = int(F.info.item())
+        if info:
+            self.sim.log.note(f"WARNING the LU met an exactly zero pivot at {info - 1}: the matrix is singular")
+
+    def _solve_kept(self, f, Z, dsc, B, t_fresh):
+        """Mixed precision with complex64 factors kept on the GPU (fast.py:996-1084)."""
+        sim = self.sim
+        n = Z.shape[0]
+        Z64 = torch.empty((n, n), dtype=dv.c64, device=Z.device)
+        dv.scale_c64(Z, dsc, Z64)                               # Z <- D Z D (kept, FP64), and its complex64 copy
+        A = lambda V: dv.zgemm_nk(Z, V)
+        k = self._reusable(f)
+        if k is not None:
+            f0, F0, d0, tf0 = k
+            r = (d0 / dsc)[:, None]
+            cap = int(np.clip(0.8 * tf0 / max(self._t_iter or 1e-3, 1e-6), *REUSE_ITER))
+            with sim.log.timer('refinement (GMRES, reused factorisation)', f"the one of {f0 / 1e9:.4g} GHz, at most {cap}"):
+                t = time.time()
+                Y, ok, rel = self._refine(A, B, lambda R: r * dv.getrs(F0, r * R, trans=1), cap, 1)
+                self._t_iter = (time.time() - t) / max(1, max(self.iters, default=1))
+            self._reach_learn(abs(np.log(f / f0)), ok, self.iters, cap)
+            if ok:
+                sim.log.note(f"reused LU ({f0 / 1e9:.4g} GHz): GMRES iterations per port {self.iters}")
+                del Z64
+                return dsc[:, None] * Y
+            sim.log.note(f"the LU of {f0 / 1e9:.4g} GHz did not precondition {f / 1e9:.4g} GHz in {cap} iterations "
+                         f"(residual {rel:.1e}) - a new one")
+        with sim.log.timer('LU factor (GPU)', f"N={n}, complex64"):
+            F = dv.getrf(Z64)
+        self._check_info(F)
+        with sim.log.timer('refinement (GMRES, dense double)'):
+            t = time.time()
+            Y, ok, rel = self._refine(A, B, lambda R: dv.getrs(F, R, trans=1), MIXED_MAX_ITER, 1, FRESH_STALL)
+            self._t_iter = (time.time() - t) / max(1, max(self.iters, default=1))
+        sim.log.note(f"mixed precision (GPU): GMRES iterations per port {self.iters}")
+        if ok:
+            if sim.reuse != 'off':
+                self._kept.append((f, F, dsc, time.time() - t_fresh))
+                while len(self._kept) > REUSE_KEEP:
+                    self._drop_kept()
+            return dsc[:, None] * Y
+        del F, Z64
+        sim.log.note(f"WARNING mixed precision did not converge at {f / 1e9:.4g} GHz (residual {rel:.1e}) - solving "
+                     f"in double precision up to {MIXED_RETRY * f / 1e9:.4g} GHz")
+        self._mixed_failed = max(self._mixed_failed, f)
+        with sim.log.timer('LU factor', f"N={n}, complex128, GPU"):
+            Fd = dv.getrf(Z)                                     # Z is equilibrated already
+        return dsc[:, None] * dv.getrs(Fd, B, trans=1)

@@ -33,3 +33,31 @@ def seeded_matrix(n, seed=3, swap=True, dtype=np.complex128):
     if swap and n > 7:
         Z[[0, 7]] = Z[[7, 0]]
     return Z
+
+
+def oracle_board(sim):
+    """The oracle's plain description (oracle.solve.Board) of a product Simulation: the same triangles, layers,
+    chains, fans and excitation; the oracle builds its own basis, geometry and tables from them."""
+    from oracle import greens as og
+    from oracle.solve import Board
+    st = sim.st
+    ost = og.LayerStack([og.Layer(l.h, l.er, l.tand) for l in st.layers], top=st.top, top_er=st.top_er,
+                        bottom=st.bottom, bottom_er=st.bottom_er)
+    chains = [(v.x, v.y, v.a, kb, kt, id(v), v.Z) for v, kb, kt in sim.chains]
+    lumped = [(idx, sig, el.Z) for el, (idx, sig) in zip(sim.lumped, sim.lumped_edges)]
+    return Board(stack=ost, nodes=sim.rwg.nodes, tris=sim.rwg.tris, tri_layer=sim.tri_layer, layers=sim.layers,
+                 dmax=sim.dmax, chains=chains, F=sim.F, dF=sim.dF, E=sim.excitation(), lumped=lumped,
+                 near_fac=sim.near_fac)
+
+
+def port_rows(sim):
+    """Each port's current rows and signs (sim.port_current)."""
+    from mom25d_amd.sim import ViaPort
+    rows, signs = [], []
+    for j, p in enumerate(sim.ports):
+        if isinstance(p, ViaPort):
+            rows.append(np.array([sim.Ne + sim._port_chain(p)])); signs.append(np.array([p.sign]))
+        else:
+            idx, sig = sim.gap[j]
+            rows.append(idx); signs.append(sig * sim.rwg.l[idx])
+    return rows, signs

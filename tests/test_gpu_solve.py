@@ -1,0 +1,109 @@
+"""The whole per-frequency solve on the GPU against the oracle's CPU restatement (tables, Z, S) on the
+configuration boards that the oracle finishes in seconds, and size-independent properties on the 8k-unknown
+board.
+
+Tolerances: Z - row-relative 1e-12, and per-entry 1e-8 above 1e-4 of the row's largest entry. The fill kernels are
+bit-exact given the same tables (test_gpu_kernels.py); here the tables are the GPU's own (the Hankel sums' J0 and
+the spline's order of operations round differently from glibc / SciPy: the tables agree to 1e-12 of their largest
+value), and far entries whose four half-pair terms cancel by up to 1e4 carry that difference to 2.5e-9 relative.
+S - |dS| <= 1e-5 for the FP64 path, <= 1e-4 for the mixed path after refinement (the north star's bounds).
+"""
+import numpy as np
+import pytest
+
+from mom25d_amd import boards
+from mom25d_amd.sim import Simulation
+from oracle import greens as og
+from oracle.solve import Solver, sparams as osparams
+from tests.cases import oracle_board, port_rows
+
+pytestmark = pytest.mark.gpu
+
+
+def make(board, precision, **kw):
+    st, sh, po, vi, bkw = board()
+    bkw.update(kw)
+    return Simulation(st, sh, po, vi, precision=precision, verbose=False, **bkw)
+
+
+@pytest.fixture(scope='module')
+def c1(gpu):
+    sim = make(boards.microstrip, 'double')
+    return sim, Solver(oracle_board(sim))
+
+
+def rel_row(a, b):
+    return float(np.max(np.abs(a - b) / np.abs(b).max(axis=-1, keepdims=True)))
+
+
+def rel_entry(a, b, floor):
+    fl = floor * np.abs(b).max(axis=-1, keepdims=True)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), fl)))
+
+
+def test_tables_match_oracle(c1):
+    sim, osv = c1
+    f = 3.3e9
+    tg = sim.green_tables(f)
+    to = og.green_tables(f, osv.b.stack, osv.b.layers, osv.rho_max, osv.b.dmax)
+    for key in to:
+        a, b = tg[key], to[key]
+        for x, y in ((a.A_rest.cpu().numpy(), b.A_rest), (a.Q_rest.cpu().numpy(), b.Q_rest)):
+            assert np.abs(x - y).max() <= 1e-12 * np.abs(y).max()
+        for name in ('cA', 'cq', 'cAi', 'cqi', 'dimg', 'd'):
+            assert abs(getattr(a, name) - getattr(b, name)) <= 1e-13 * max(abs(getattr(b, name)), 1e-300), name
+
+
+def test_fill_matches_oracle(c1):
+    sim, osv = c1
+    f = 3.3e9
+    Zg = sim.fast.fill(f).cpu().numpy()
+    Zo = osv.fill(f)
+    assert rel_row(Zg, Zo) < 1e-12
+    assert rel_entry(Zg, Zo, 1e-4) < 1e-8
+
+
+def test_microstrip_s_double_and_mixed_match_oracle(c1):
+    sim, osv = c1
+    rows, signs = port_rows(sim)
+    zc = [p.zc for p in sim.ports]
+    for f in (1e9, 3.5e9, 6e9):
+        So = osparams(osv.solve(f), osv.b.E, zc, rows, signs)
+        Sd = sim.sparams(f)
+        assert np.abs(Sd - So).max() <= 1e-5, (f, np.abs(Sd - So).max())
+        assert np.abs(Sd - Sd.T).max() < 1e-8                           # reciprocal
+    sm = make(boards.microstrip, 'mixed')
+    for f in (1e9, 3.5e9):
+        So = osparams(osv.solve(f), osv.b.E, zc, rows, signs)
+        Sm = sm.sparams(f)
+        assert np.abs(Sm - So).max() <= 1e-4, (f, np.abs(Sm - So).max())
+    assert any('mixed precision' in n or 'reused' in n for n in sm.log.notes)
+
+
+def test_coupled_lines_match_oracle(gpu):
+    sim = make(boards.coupled_lines, 'double')
+    osv = Solver(oracle_board(sim))
+    rows, signs = port_rows(sim)
+    zc = [p.zc for p in sim.ports]
+    f = 5e9
+    So = osparams(osv.solve(f), osv.b.E, zc, rows, signs)
+    Sd = sim.sparams(f)
+    assert np.abs(Sd - So).max() <= 1e-5
+    Sm = make(boards.coupled_lines, 'mixed').sparams(f)
+    assert np.abs(Sm - So).max() <= 1e-4
+
+
+def test_seeded_8k_board_double_vs_mixed(gpu):
+    """C2 (N ~ 7.8k, vias between two interfaces, 4 via ports): mixed agrees with double to 1e-4, reciprocal and
+    passive; a reused factor at a nearby frequency gives the fresh answer."""
+    sd = make(boards.seeded_board, 'double')
+    sm = make(boards.seeded_board, 'mixed')
+    f = 4e9
+    Sd, Sm = sd.sparams(f), sm.sparams(f)
+    assert np.abs(Sm - Sd).max() <= 1e-4
+    assert np.abs(Sd - Sd.T).max() < 1e-6
+    assert np.linalg.svd(Sd, compute_uv=False).max() < 1 + 1e-6
+    f2 = 4.2e9
+    Sr = sm.sparams(f2)
+    assert any('reused' in n for n in sm.log.notes)
+    assert np.abs(Sr - sd.sparams(f2)).max() <= 1e-4
