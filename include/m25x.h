@@ -152,12 +152,19 @@ int m25x_ipiv_perm(const int *ipiv, long n, int *perm, void *stream);
 /* Device workspace bytes m25x_getrs needs. */
 size_t m25x_getrs_work_bytes(int dtype, long n, long ldb);
 
+/* Device bytes of the inverted diagonal blocks of a factor (two 64 x 64 blocks per 64 rows). */
+size_t m25x_dinv_bytes(int dtype, long n);
+
+/* The inverses of the factor's 64 x 64 diagonal blocks (L_bb^-1 and U_bb^-1), once per factor, for m25x_getrs. */
+int m25x_trtri_blocks(int dtype, const void *LU, long n, long lda, void *dinv, void *stream);
+
 /* Solves with the factor of m25x_getrf: trans = 1: Z X = B (A^T X = B, lu_solve(..., trans=1)), trans = 0:
  * A X = B. B: n x nrhs row-major (leading dimension ldb >= nrhs, 1 <= nrhs <= 8), the factor's dtype, overwritten
- * with X. perm: m25x_ipiv_perm's (or NULL: computed here). Replaces the backend's solve (src/mom25d/gpu.py:76-80;
- * src/mom25d/gpu_vk.py:650-678) and zgetrs / cgetrs (src/mom25d/fast.py:356-358). */
-int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B, long ldb,
-               int nrhs, int trans, void *work, size_t work_bytes, void *stream);
+ * with X. perm: m25x_ipiv_perm's (or NULL: computed here); dinv: m25x_trtri_blocks' (or NULL: computed here).
+ * Replaces the backend's solve (src/mom25d/gpu.py:76-80; src/mom25d/gpu_vk.py:650-678) and zgetrs / cgetrs
+ * (src/mom25d/fast.py:356-358). */
+int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, const void *dinv,
+               void *B, long ldb, int nrhs, int trans, void *work, size_t work_bytes, void *stream);
 
 /* ---- refinement products ------------------------------------------------------------------------------- */
 

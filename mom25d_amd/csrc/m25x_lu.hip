@@ -11,7 +11,8 @@
 //   gemm    A22 -= A21 U12 on the matrix cores: complex products as real ones through the real embedding
 //           (Re C = [Re A, Im A] [Re B; -Im B], Im C = [Re A, Im A] [Im B; Re B]), v_mfma_f64_16x16x4 for
 //           complex128 and v_mfma_f32_16x16x4 for complex64 (FP32 accumulation)
-// The solves (getrs) run one workgroup per 64-row block with ready flags between blocks (no launch per block).
+// The solves (getrs) run one workgroup per 64-row block with ready flags between blocks (no launch per block), each
+// block's diagonal solve a product with its inverse (k_trtri_diag, once per factor).
 #include <algorithm>
 #include <mutex>
 #include "m25x_common.h"
@@ -402,48 +403,105 @@ __global__ void k_perm(const int *ipiv, long n, int *perm)
 
 // ---- triangular solves, one workgroup a 64-row block, ready flags between blocks ---------------------------
 // B: n x nrhs row-major (ldb). The blocks of the solution are published in order; block b adds the
-// contributions of blocks 0..b-1 as their flags come up, then solves its diagonal block.
+// contributions of the blocks before it as their flags come up, then multiplies by the inverse of its diagonal
+// block (inverted once per factor by k_trtri_diag): no serial substitution on the critical path.
 constexpr int SB = 64;
 constexpr int MAXRHS = 8;
 
-// lower: U^T (trans, forward) with U[k][i] = LU[i * lda + k], non-unit; or L (no trans, forward), unit.
-// upper: L^T (trans, backward) with L[k][i] = LU[i * lda + k], unit; or U (no trans, backward), non-unit.
-// kind 0: U^T y = b; 1: L^T z = y; 2: L y = b; 3: U x = y.
+// Per diagonal block b: Dinv[b][0] = L_bb^-1 (unit lower), Dinv[b][1] = U_bb^-1 (upper), 64 x 64 row-major;
+// a last block shorter than 64 is padded with the identity. One workgroup a block, one column a thread.
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsv(const T *LU, long lda, long n, T *B, long ldb, int nrhs, int kind,
-                                              int *flags, int *abort)
+__global__ __launch_bounds__(64) void k_trtri_diag(const T *LU, long lda, long n, T *Dinv)
 {
-    __shared__ T D[SB][SB + 1];          // the diagonal block, D[i][k] = the operator's entry (row i, column k)
+    __shared__ T M[SB][SB + 1];
+    __shared__ T X[SB][SB + 1];
+    const long i0 = (long)blockIdx.x * SB;
+    const int bn = (int)std::min((long)SB, n - i0);
+    const int j = threadIdx.x;
+    const T zero = CX<T>::make(0, 0), one = CX<T>::make(1, 0);
+    for (int which = 0; which < 2; ++which) {
+        // M[i][k]: L (which 0, strictly lower part, unit diagonal) or U (which 1, upper part) of the block
+        for (int i = 0; i < SB; ++i) {
+            T v;
+            if (i < bn && j < bn) {
+                const T a = LU[(i0 + j) * lda + i0 + i];     // A[i0+i][i0+j], column-major
+                v = which == 0 ? (j < i ? a : (j == i ? one : zero)) : (j >= i ? a : zero);
+            } else {
+                v = (i == j) ? one : zero;
+            }
+            M[i][j] = v;
+        }
+        __syncthreads();
+        if (which == 0) {               // forward: X[i][j] = -sum_{j <= k < i} L[i][k] X[k][j]
+            for (int i = 0; i < SB; ++i) {
+                T s = i == j ? one : zero;
+                if (i > j) {
+                    for (int k = j; k < i; ++k) s = tsub(s, tmul(M[i][k], X[k][j]));
+                }
+                X[i][j] = i < j ? zero : s;
+            }
+        } else {                        // backward: X[i][j] = (delta_ij - sum_{i < k <= j} U[i][k] X[k][j]) / U[i][i]
+            for (int i = SB - 1; i >= 0; --i) {
+                T s = i == j ? one : zero;
+                if (i <= j) {
+                    for (int k = i + 1; k <= j; ++k) s = tsub(s, tmul(M[i][k], X[k][j]));
+                    X[i][j] = tdiv(s, M[i][i]);
+                } else {
+                    X[i][j] = zero;
+                }
+            }
+        }
+        __syncthreads();
+        T *out = Dinv + ((size_t)blockIdx.x * 2 + which) * SB * SB;
+        for (int i = 0; i < SB; ++i) out[i * SB + j] = X[i][j];
+        __syncthreads();
+    }
+}
+
+// kind 0: U^T y = b (forward); 1: L^T z = y (backward); 2: L y = b (forward); 3: U x = y (backward).
+// Operator entry (row i, column k): kinds 0, 1 read LU[i * lda + k] (U[k][i], L[k][i]); kinds 2, 3 LU[k * lda + i].
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv(const T *LU, long lda, long n, const T *Dinv, T *B, long ldb, int nrhs,
+                                              int kind, int *flags, int *abort)
+{
+    __shared__ T Mi[SB][SB + 1];             // the diagonal block's inverse for this operator, Mi[i][k]
     __shared__ T acc[SB][MAXRHS];
-    __shared__ T part[2][SB][MAXRHS];
+    __shared__ T part[4][SB][MAXRHS];
     const int nb = (int)((n + SB - 1) / SB);
     const bool fwd = kind == 0 || kind == 2;
-    const int b = fwd ? blockIdx.x : nb - 1 - blockIdx.x;     // the order of publication is blockIdx order
+    const int b = fwd ? blockIdx.x : nb - 1 - blockIdx.x;     // publication order is blockIdx order
     const long i0 = (long)b * SB;
     const int bn = (int)std::min((long)SB, n - i0);
     const int tid = threadIdx.x;
-    // operator entry (row i, column k), global indices
-    auto op = [&](long i, long k) -> T {
-        switch (kind) {
-        case 0: return LU[i * lda + k];      // U^T[i][k] = U[k][i]
-        case 1: return LU[i * lda + k];      // L^T[i][k] = L[k][i]
-        case 2: return LU[k * lda + i];      // L[i][k]
-        default: return LU[k * lda + i];     // U[i][k]
+    const bool rowmajor = kind <= 1;
+    {
+        // kind 0: (U^-1)^T, 1: (L^-1)^T, 2: L^-1, 3: U^-1
+        const T *D = Dinv + ((size_t)b * 2 + ((kind == 0 || kind == 3) ? 1 : 0)) * SB * SB;
+        const bool tr = kind <= 1;
+        for (int idx = tid; idx < SB * SB; idx += 256) {
+            const int i = idx / SB, k = idx % SB;
+            Mi[i][k] = tr ? D[k * SB + i] : D[i * SB + k];
         }
-    };
-    for (int idx = tid; idx < SB * SB; idx += 256) {
-        const int i = idx / SB, k = idx % SB;
-        D[i][k] = (i < bn && k < bn) ? op(i0 + i, i0 + k) : CX<T>::make(0, 0);
     }
     for (int idx = tid; idx < SB * MAXRHS; idx += 256) {
         const int i = idx / MAXRHS, r = idx % MAXRHS;
         acc[i][r] = (i < bn && r < nrhs) ? B[(i0 + i) * ldb + r] : CX<T>::make(0, 0);
     }
     __syncthreads();
-    // contributions of the blocks before this one, in publication order
-    const int row = tid & 63, ks = (tid >> 6) & 1;   // 2 k-slices of 32 (threads 128-255 idle here)
+    const int row = tid & 63, kq = tid >> 6;                  // 4 slices of 16 columns
     for (int q = 0; q < (int)blockIdx.x; ++q) {
         const int a = fwd ? q : nb - 1 - q;
+        const long k0 = (long)a * SB;
+        const int kn = (int)std::min((long)SB, n - k0);
+        // this thread's 16 operator entries of block (b, a), loaded before the wait
+        T m[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int kx = kq * 16 + u;
+            const bool in = row < bn && kx < kn;
+            const long gi = i0 + row, gk = k0 + kx;
+            m[u] = in ? (rowmajor ? LU[gi * lda + gk] : LU[gk * lda + gi]) : CX<T>::make(0, 0);
+        }
         if (tid == 0) {
             long spins = 0;
             while (__hip_atomic_load(flags + a, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
@@ -452,49 +510,40 @@ __global__ __launch_bounds__(256) void k_trsv(const T *LU, long lda, long n, T *
             }
         }
         __syncthreads();
-        const long k0 = (long)a * SB;
-        const int kn = (int)std::min((long)SB, n - k0);
         T s[MAXRHS];
 #pragma unroll
         for (int r = 0; r < MAXRHS; ++r) s[r] = CX<T>::make(0, 0);
-        if (row < bn && tid < 128) {
-            for (int kx = ks * 32; kx < std::min(kn, ks * 32 + 32); ++kx) {
-                const T m = op(i0 + row, k0 + kx);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int kx = kq * 16 + u;
+            if (kx < kn) {
                 const T *y = B + (k0 + kx) * ldb;
-                for (int r = 0; r < nrhs; ++r) s[r] = tadd(s[r], tmul(m, y[r]));
+#pragma unroll
+                for (int r = 0; r < MAXRHS; ++r)
+                    if (r < nrhs) s[r] = tadd(s[r], tmul(m[u], y[r]));
             }
         }
-        if (tid < 128)
-            for (int r = 0; r < nrhs; ++r) part[ks][row][r] = s[r];
+#pragma unroll
+        for (int r = 0; r < MAXRHS; ++r) part[kq][row][r] = s[r];
         __syncthreads();
         for (int idx = tid; idx < SB * nrhs; idx += 256) {
             const int i = idx / nrhs, r = idx % nrhs;
-            acc[i][r] = tsub(acc[i][r], tadd(part[0][i][r], part[1][i][r]));
+            acc[i][r] = tsub(acc[i][r], tadd(tadd(part[0][i][r], part[1][i][r]), tadd(part[2][i][r], part[3][i][r])));
         }
         __syncthreads();
     }
-    // the diagonal block: column-oriented substitution (forward or backward)
-    const bool unit = kind == 1 || kind == 2;
-    for (int st = 0; st < bn; ++st) {
-        const int k = fwd ? st : bn - 1 - st;
-        if (!unit) {
-            if (tid < nrhs) acc[k][tid] = tdiv(acc[k][tid], D[k][k]);
-            __syncthreads();
-        }
-        for (int idx = tid; idx < SB * nrhs; idx += 256) {
-            const int i = idx / nrhs, r = idx % nrhs;
-            const bool after = fwd ? (i > k && i < bn) : (i < k);
-            if (after) acc[i][r] = tsub(acc[i][r], tmul(D[i][k], acc[k][r]));
-        }
-        __syncthreads();
-    }
+    // the diagonal block: y = Mi acc
     for (int idx = tid; idx < bn * nrhs; idx += 256) {
         const int i = idx / nrhs, r = idx % nrhs;
-        B[(i0 + i) * ldb + r] = acc[i][r];
+        T v = CX<T>::make(0, 0);
+        for (int k = 0; k < bn; ++k) v = tadd(v, tmul(Mi[i][k], acc[k][r]));
+        B[(i0 + i) * ldb + r] = v;
     }
-    __threadfence();
     __syncthreads();
-    if (tid == 0) __hip_atomic_store(flags + b, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) {
+        __threadfence();
+        __hip_atomic_store(flags + b, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // out[perm[i]] = in[i] (scatter, trans) or out[i] = in[perm[i]] (gather)
@@ -578,15 +627,21 @@ int getrf_t(T *A, long n, long lda, int *ipiv, int *info, void *work, size_t wby
     return M25X_OK;
 }
 
+template <typename T> size_t dinv_bytes(long n)
+{
+    const long nb = (n + SB - 1) / SB;
+    return (size_t)nb * 2 * SB * SB * sizeof(T);
+}
+
 template <typename T> size_t getrs_work(long n, long ldb)
 {
     const long nb = (n + SB - 1) / SB;
-    return (size_t)n * ldb * sizeof(T) + sizeof(int) * ((size_t)n + 2 * nb + 64);
+    return (size_t)n * ldb * sizeof(T) + dinv_bytes<T>(n) + sizeof(int) * ((size_t)n + 2 * nb + 64);
 }
 
 template <typename T>
-int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, T *B, long ldb, int nrhs, int trans,
-            void *work, size_t wbytes, hipStream_t s)
+int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, const T *dinv, T *B, long ldb,
+            int nrhs, int trans, void *work, size_t wbytes, hipStream_t s)
 {
     if (nrhs < 1 || nrhs > MAXRHS) return fail(M25X_EINVAL, "getrs: 1 to %d right-hand sides, not %d", MAXRHS, nrhs);
     if (ldb < nrhs) return fail(M25X_EINVAL, "getrs: ldb < nrhs");
@@ -595,7 +650,8 @@ int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, 
     const int nb = (int)((n + SB - 1) / SB);
     const size_t tmp_bytes = (size_t)n * ldb * sizeof(T);
     T *tmp = (T *)work;
-    int *perm = (int *)((char *)work + tmp_bytes);
+    T *dwork = (T *)((char *)work + tmp_bytes);
+    int *perm = (int *)((char *)dwork + dinv_bytes<T>(n));
     int *flags = perm + n;
     int *abort = flags + 2 * nb;
     hipLaunchKernelGGL(k_zero_ints, dim3(blocks(2 * nb + 1, 256)), dim3(256), 0, s, flags, 2 * nb + 1);
@@ -603,17 +659,21 @@ int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, 
         hipLaunchKernelGGL(k_perm, dim3(1), dim3(256), 0, s, ipiv, n, perm);
         perm_in = perm;
     }
+    if (!dinv) {
+        hipLaunchKernelGGL(k_trtri_diag<T>, dim3(nb), dim3(SB), 0, s, LU, lda, n, dwork);
+        dinv = dwork;
+    }
     if (int rc = check_launch("getrs")) return rc;
     if (trans) {
-        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, B, ldb, nrhs, 0, flags, abort);
-        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, B, ldb, nrhs, 1, flags + nb, abort);
+        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, dinv, B, ldb, nrhs, 0, flags, abort);
+        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, dinv, B, ldb, nrhs, 1, flags + nb, abort);
         hipLaunchKernelGGL(k_permute<T>, dim3(blocks(n * nrhs, 256)), dim3(256), 0, s, (const T *)B, tmp, n, nrhs, ldb,
                            perm_in, 1);
     } else {
         hipLaunchKernelGGL(k_permute<T>, dim3(blocks(n * nrhs, 256)), dim3(256), 0, s, (const T *)B, tmp, n, nrhs, ldb,
                            perm_in, 0);
-        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, tmp, ldb, nrhs, 2, flags, abort);
-        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, tmp, ldb, nrhs, 3, flags + nb, abort);
+        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, dinv, tmp, ldb, nrhs, 2, flags, abort);
+        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, dinv, tmp, ldb, nrhs, 3, flags + nb, abort);
     }
     if (int rc = check_launch("getrs")) return rc;
     M25X_HIP(hipMemcpyAsync(B, tmp, tmp_bytes, hipMemcpyDeviceToDevice, s));
@@ -653,18 +713,38 @@ int m25x_ipiv_perm(const int *ipiv, long n, int *perm, void *stream)
     return check_launch("ipiv_perm");
 }
 
-int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B, long ldb,
-               int nrhs, int trans, void *work, size_t work_bytes, void *stream)
+size_t m25x_dinv_bytes(int dtype, long n)
+{
+    return dtype == M25X_C64 ? dinv_bytes<float2>(n) : dinv_bytes<double2>(n);
+}
+
+int m25x_trtri_blocks(int dtype, const void *LU, long n, long lda, void *dinv, void *stream)
+{
+    if (!LU || !dinv || n < 0 || lda < n) return fail(M25X_EINVAL, "trtri_blocks: bad arguments");
+    if (n == 0) return M25X_OK;
+    const unsigned nb = (unsigned)((n + SB - 1) / SB);
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == M25X_C128)
+        hipLaunchKernelGGL(k_trtri_diag<double2>, dim3(nb), dim3(SB), 0, s, (const double2 *)LU, lda, n, (double2 *)dinv);
+    else if (dtype == M25X_C64)
+        hipLaunchKernelGGL(k_trtri_diag<float2>, dim3(nb), dim3(SB), 0, s, (const float2 *)LU, lda, n, (float2 *)dinv);
+    else
+        return fail(M25X_EINVAL, "trtri_blocks: dtype %d", dtype);
+    return check_launch("trtri_blocks");
+}
+
+int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, const void *dinv,
+               void *B, long ldb, int nrhs, int trans, void *work, size_t work_bytes, void *stream)
 {
     if (!LU || !ipiv || !B || n < 0 || lda < n) return fail(M25X_EINVAL, "getrs: bad arguments");
     if (n == 0) return M25X_OK;
     hipStream_t s = (hipStream_t)stream;
     if (dtype == M25X_C128)
-        return getrs_t<double2>((const double2 *)LU, n, lda, ipiv, perm, (double2 *)B, ldb, nrhs, trans, work,
-                                work_bytes, s);
+        return getrs_t<double2>((const double2 *)LU, n, lda, ipiv, perm, (const double2 *)dinv, (double2 *)B, ldb, nrhs,
+                                trans, work, work_bytes, s);
     if (dtype == M25X_C64)
-        return getrs_t<float2>((const float2 *)LU, n, lda, ipiv, perm, (float2 *)B, ldb, nrhs, trans, work,
-                               work_bytes, s);
+        return getrs_t<float2>((const float2 *)LU, n, lda, ipiv, perm, (const float2 *)dinv, (float2 *)B, ldb, nrhs,
+                               trans, work, work_bytes, s);
     return fail(M25X_EINVAL, "getrs: dtype %d", dtype);
 }
 

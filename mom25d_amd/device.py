@@ -270,8 +270,8 @@ class Factor:
     """An LU factor on the device: A = Z^T (column-major view of the row-major buffer) = P L U in place, ipiv
     0-based, perm its permutation; dtype complex128 or complex64."""
 
-    def __init__(self, LU, ipiv, perm, info):
-        self.LU, self.ipiv, self.perm, self.info = LU, ipiv, perm, info
+    def __init__(self, LU, ipiv, perm, info, dinv=None):
+        self.LU, self.ipiv, self.perm, self.info, self.dinv = LU, ipiv, perm, info, dinv
         self.n = LU.shape[0]
         self.dtype = LU.dtype
 
@@ -293,7 +293,9 @@ def getrf(Z):
     L.call('m25x_getrf', dt, ptr(Z), n, Z.stride(0), ptr(ipiv), ptr(info), ptr(work), wb, stream())
     perm = torch.empty(n, dtype=i32, device=dv)
     L.call('m25x_ipiv_perm', ptr(ipiv), n, ptr(perm), stream())
-    return Factor(Z, ipiv, perm, info)
+    dinv = torch.empty(L.lib().m25x_dinv_bytes(dt, n), dtype=torch.uint8, device=dv)
+    L.call('m25x_trtri_blocks', dt, ptr(Z), n, Z.stride(0), ptr(dinv), stream())
+    return Factor(Z, ipiv, perm, info, dinv)
 
 
 def getrs(F, B, trans=1):
@@ -309,8 +311,8 @@ def getrs(F, B, trans=1):
         X = B[:, c0:c0 + kc].to(F.dtype).contiguous()
         wb = L.lib().m25x_getrs_work_bytes(dt, n, kc)
         work = torch.empty(wb, dtype=torch.uint8, device=B.device)
-        L.call('m25x_getrs', dt, ptr(F.LU), n, F.LU.stride(0), ptr(F.ipiv), ptr(F.perm), ptr(X), kc, kc, int(trans),
-               ptr(work), wb, stream())
+        L.call('m25x_getrs', dt, ptr(F.LU), n, F.LU.stride(0), ptr(F.ipiv), ptr(F.perm), ptr(F.dinv), ptr(X), kc, kc,
+               int(trans), ptr(work), wb, stream())
         out[:, c0:c0 + kc] = X.to(c128)
     return out[:, 0] if one else out
 
