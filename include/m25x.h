@@ -133,16 +133,19 @@ int m25x_diag_scales(const double *Z, long n, long ld, double *d, void *stream);
 
 /* ---- dense complex LU ---------------------------------------------------------------------------------- */
 
-/* Device workspace bytes m25x_getrf needs for an n x n matrix of the given dtype. */
+/* Device workspace bytes m25x_getrf needs for an n x n matrix of the given dtype (either pivoting). */
 size_t m25x_getrf_work_bytes(int dtype, long n);
 
-/* LU with partial pivoting of the column-major n x n matrix A (leading dimension lda; the row-major Z seen as
- * Z^T), in place: L unit lower, U upper, ipiv 0-based LAPACK interchanges (int32, device), the layout of
+#define M25X_PIVOT_PARTIAL 0      /* LAPACK's partial pivoting: the pivots of zgetrf / cgetrf exactly */
+#define M25X_PIVOT_TOURNAMENT 1   /* tournament pivoting (CALU): a panel's pivots by a tree of GEPPs */
+
+/* LU of the column-major n x n matrix A (leading dimension lda; the row-major Z seen as Z^T), in place: L unit
+ * lower, U upper, ipiv 0-based LAPACK-style interchanges (int32, device), the layout of
  * scipy.linalg.lu_factor(Z.T). complex128 (trailing updates on FP64 MFMA) or complex64 (FP32 MFMA, FP32
- * accumulation). info (device int, may be NULL): 0, or 1 + the first zero pivot. Replaces the GPU backend's
- * factor (src/mom25d/gpu.py:69-74; src/mom25d/gpu_vk.py:515-633) and LAPACK zgetrf/cgetrf
- * (src/mom25d/fast.py:318-320). */
-int m25x_getrf(int dtype, void *A, long n, long lda, int *ipiv, int *info, void *work, size_t work_bytes,
+ * accumulation). pivot: M25X_PIVOT_PARTIAL or M25X_PIVOT_TOURNAMENT. info (device int, may be NULL): 0, or 1 + the
+ * first zero pivot. Replaces the GPU backend's factor (src/mom25d/gpu.py:69-74; src/mom25d/gpu_vk.py:515-633) and
+ * LAPACK zgetrf/cgetrf (src/mom25d/fast.py:318-320). */
+int m25x_getrf(int dtype, int pivot, void *A, long n, long lda, int *ipiv, int *info, void *work, size_t work_bytes,
                void *stream);
 
 /* The interchanges of ipiv as one permutation (perm = ipiv's swaps applied in order to 0..n-1), once per factor:

@@ -8,6 +8,7 @@ LIB_PATH = os.path.join(_HERE, 'libm25x.so')
 
 C128, C64 = 0, 1
 EINVAL, EHIP, ENOMEM, ESINGULAR = -1, -2, -3, -4
+PIVOT_PARTIAL, PIVOT_TOURNAMENT = 0, 1
 
 
 class Tables(C.Structure):
@@ -45,7 +46,7 @@ _SIGS = {
     'm25x_scale_c64': ([P, L, L, P, P, L, P], I),
     'm25x_diag_scales': ([P, L, L, P, P], I),
     'm25x_getrf_work_bytes': ([I, L], S),
-    'm25x_getrf': ([I, P, L, L, P, P, P, S, P], I),
+    'm25x_getrf': ([I, I, P, L, L, P, P, P, S, P], I),
     'm25x_ipiv_perm': ([P, L, P, P], I),
     'm25x_getrs_work_bytes': ([I, L, L], S),
     'm25x_dinv_bytes': ([I, L], S),

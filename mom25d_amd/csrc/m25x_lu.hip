@@ -1,7 +1,8 @@
 // m25x_lu.hip - dense complex LU with partial pivoting on gfx950, and the triangular solves with its factor.
 //
 // A is column-major (the row-major Z of the reference seen as A = Z^T, the layout of scipy's lu_factor(Z.T)).
-// Blocked right-looking, panels of NB columns:
+// Blocked right-looking. Two pivoting strategies:
+// Partial pivoting (LAPACK's pivots exactly), panels of NB columns:
 //   panel   one cooperative launch per panel: each workgroup keeps a slab of the panel's rows in LDS; every column
 //           costs one grid barrier (pivot search, row interchange, scaling and the panel's rank-1 update in
 //           between, the next column's candidates published before the barrier); pivots by LAPACK's rule
@@ -11,6 +12,9 @@
 //   gemm    A22 -= A21 U12 on the matrix cores: complex products as real ones through the real embedding
 //           (Re C = [Re A, Im A] [Re B; -Im B], Im C = [Re A, Im A] [Im B; Re B]), v_mfma_f64_16x16x4 for
 //           complex128 and v_mfma_f32_16x16x4 for complex64 (FP32 accumulation)
+// Tournament pivoting (CALU), panels of TW = 64 columns: the panel's pivots by a tree of GEPPs in one launch with no
+// grid barrier (k_tslu), its row moves on every column (k_apply_moves), then L21 = A21 U11^-1, U12 = L11^-1 A12 and
+// A22 -= L21 U12, all three on the matrix cores.
 // The solves (getrs) run one workgroup per 64-row block with ready flags between blocks (no launch per block), each
 // block's diagonal solve a product with its inverse (k_trtri_diag, once per factor).
 #include <algorithm>
@@ -294,15 +298,17 @@ __global__ __launch_bounds__(256) void k_trsm(T *A, long lda, long n, long j0, i
     }
 }
 
-// ---- C -= A B on the matrix cores --------------------------------------------------------------------------
-// C (M x N), A (M x K), B (K x N), column-major, complex. Workgroup tile 64 x 64, four waves of 32 x 32, each
+// ---- C -= A B (SUB) or C = A B on the matrix cores -------------------------------------------------------
+// C (M x N), A (M x K), B (K x N), column-major, complex. C = A B may overwrite A or B in place when a workgroup
+// tile's output depends on nothing another workgroup writes (M <= 64 with C = B, or N <= 64 with C = A): every
+// workgroup reads all of its K range before it stores. Workgroup tile 64 x 64, four waves of 32 x 32, each
 // 2 x 2 blocks of 16 x 16 with a real and an imaginary accumulator. The MFMA's first operand is B (its rows are
 // C's columns), its second A, so a lane's outputs run down C's columns (coalesced stores).
 constexpr int GM = 64, GN = 64, GK = 8;
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_gemm_sub(T *C, long ldc, const T *A, long lda, const T *B, long ldb, long M,
-                                                  long N, long K)
+template <typename T, bool SUB>
+__global__ __launch_bounds__(256) void k_gemm(T *C, long ldc, const T *A, long lda, const T *B, long ldb, long M,
+                                              long N, long K)
 {
     typedef typename CX<T>::R R;
     typedef typename CX<T>::acc4 acc4;
@@ -368,8 +374,12 @@ __global__ __launch_bounds__(256) void k_gemm_sub(T *C, long ldc, const T *A, lo
                 const long gi = m0 + wm + a * 16 + li, gj = n0 + wn + b * 16 + CX<T>::out_row(kk, r);
                 if (gi < M && gj < N) {
                     T *c = C + gj * ldc + gi;
-                    const T v = *c;
-                    *c = CX<T>::make(v.x - cr[b][a][r], v.y - ci[b][a][r]);
+                    if (SUB) {
+                        const T v = *c;
+                        *c = CX<T>::make(v.x - cr[b][a][r], v.y - ci[b][a][r]);
+                    } else {
+                        *c = CX<T>::make(cr[b][a][r], ci[b][a][r]);
+                    }
                 }
             }
 }
@@ -408,6 +418,54 @@ __global__ void k_perm(const int *ipiv, long n, int *perm)
 constexpr int SB = 64;
 constexpr int MAXRHS = 8;
 
+// The inverse of the unit lower (which 0) or upper (which 1) triangle of the bn x bn block at LU (column-major,
+// leading dimension lda), padded to 64 x 64 with the identity; out[i * SB + j] (row-major) or out[j * SB + i]
+// (column-major). 64 threads, one column of the inverse each.
+template <typename T>
+__device__ void trtri64(const T *LU, long lda, int bn, int which, T (*M)[SB + 1], T (*X)[SB + 1], T *out,
+                        bool colmajor)
+{
+    const int j = threadIdx.x;
+    const T zero = CX<T>::make(0, 0), one = CX<T>::make(1, 0);
+    // M[i][k]: L (which 0, strictly lower part, unit diagonal) or U (which 1, upper part) of the block
+    for (int i = 0; i < SB; ++i) {
+        T v;
+        if (i < bn && j < bn) {
+            const T a = LU[(long)j * lda + i];
+            v = which == 0 ? (j < i ? a : (j == i ? one : zero)) : (j >= i ? a : zero);
+        } else {
+            v = (i == j) ? one : zero;
+        }
+        M[i][j] = v;
+    }
+    __syncthreads();
+    if (which == 0) {               // forward: X[i][j] = -sum_{j <= k < i} L[i][k] X[k][j]
+        for (int i = 0; i < SB; ++i) {
+            T s = i == j ? one : zero;
+            if (i > j) {
+                for (int k = j; k < i; ++k) s = tsub(s, tmul(M[i][k], X[k][j]));
+            }
+            X[i][j] = i < j ? zero : s;
+        }
+    } else {                        // backward: X[i][j] = (delta_ij - sum_{i < k <= j} U[i][k] X[k][j]) / U[i][i]
+        for (int i = SB - 1; i >= 0; --i) {
+            T s = i == j ? one : zero;
+            if (i <= j) {
+                for (int k = i + 1; k <= j; ++k) s = tsub(s, tmul(M[i][k], X[k][j]));
+                X[i][j] = tdiv(s, M[i][i]);
+            } else {
+                X[i][j] = zero;
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = 0; i < SB; ++i) {
+        if (colmajor) out[(long)j * SB + i] = X[i][j];
+        else out[(long)i * SB + j] = X[i][j];
+    }
+    __syncthreads();
+}
+
 // Per diagonal block b: Dinv[b][0] = L_bb^-1 (unit lower), Dinv[b][1] = U_bb^-1 (upper), 64 x 64 row-major;
 // a last block shorter than 64 is padded with the identity. One workgroup a block, one column a thread.
 template <typename T>
@@ -417,44 +475,247 @@ __global__ __launch_bounds__(64) void k_trtri_diag(const T *LU, long lda, long n
     __shared__ T X[SB][SB + 1];
     const long i0 = (long)blockIdx.x * SB;
     const int bn = (int)std::min((long)SB, n - i0);
-    const int j = threadIdx.x;
-    const T zero = CX<T>::make(0, 0), one = CX<T>::make(1, 0);
-    for (int which = 0; which < 2; ++which) {
-        // M[i][k]: L (which 0, strictly lower part, unit diagonal) or U (which 1, upper part) of the block
-        for (int i = 0; i < SB; ++i) {
-            T v;
-            if (i < bn && j < bn) {
-                const T a = LU[(i0 + j) * lda + i0 + i];     // A[i0+i][i0+j], column-major
-                v = which == 0 ? (j < i ? a : (j == i ? one : zero)) : (j >= i ? a : zero);
-            } else {
-                v = (i == j) ? one : zero;
-            }
-            M[i][j] = v;
+    for (int which = 0; which < 2; ++which)
+        trtri64(LU + i0 * lda + i0, lda, bn, which, M, X, Dinv + ((size_t)blockIdx.x * 2 + which) * SB * SB, false);
+}
+
+// ---- tournament pivoting (communication-avoiding LU panels) -----------------------------------------------
+// A panel's TW pivot rows are chosen by a tree of GEPPs (Grigori, Demmel and Xiang, "CALU: a communication
+// optimal LU factorization algorithm", SIAM J. Matrix Anal. Appl. 32 (2011)): each leaf workgroup factors RL rows
+// of the panel in LDS and nominates its pivot rows; a tree node factors its children's nominees (their original
+// rows, stacked in child order) and nominates its own; the root's pivot rows are the panel's, and the root's
+// factors of them are the panel's L11 \ U11. One launch a panel: a node is factored by whichever of its children
+// arrives last (an atomic count per node), so no workgroup ever waits on another. The rows below take
+// L21 = A21 U11^-1 and the trailing matrix the usual update, on the matrix cores.
+constexpr int TW = 64;
+template <typename T> struct Tour;
+template <> struct Tour<double2> { static constexpr int RL = 128, AR = 2; };   // rows a leaf, children a node
+template <> struct Tour<float2> { static constexpr int RL = 256, AR = 4; };
+
+template <typename T> struct TNode {
+    int id[TW];                 // the nominees' global rows, in their pivot order
+    int count, pad[63];
+    T v[TW * TW];               // their original panel rows, column-major: v[c * TW + i]
+};
+
+inline long tour_leaves(long m, int RL) { return (m + RL - 1) / RL; }
+inline long tour_nodes(long n, int RL, int AR)
+{
+    long P = tour_leaves(n, RL), tot = 0;
+    while (P > 1) { tot += P; P = (P + AR - 1) / AR; }
+    return tot + 1;
+}
+
+// GEPP of the rows in slots [0, nr) of S (column-major, S[c * R + slot], the slots in the rows' stacking order)
+// over columns [0, nc); thread t owns slot t and rows never move: a row's position in the elimination order is
+// tracked instead, and ties in the pivot search go to the earliest position (LAPACK's rule). order[k] = the slot
+// of the k-th pivot. On return a slot chosen at step k holds row k of L \ U, the other slots their multipliers
+// and Schur complement. One barrier a step: the per-wave maxima are double-buffered.
+template <typename T, int NT>
+__device__ int lds_gepp(T *S, int R, int nr, int nc, int *order, double (*wv)[NT / 64], int (*wk)[NT / 64])
+{
+    constexpr int NW = NT / 64;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int npiv = nr < nc ? nr : nc;
+    const T zero = CX<T>::make(0, 0);
+    bool live = t < nr;
+    int pos = t;
+    for (int k = 0; k < npiv; ++k) {
+        const T a = live ? S[k * R + t] : zero;
+        double v = live ? cabs1(a) : -1.0;
+        if (live && !(v >= 0)) v = 0;        // a NaN never outranks a live row, and a live row always wins over none
+        int key = live ? ((pos << 16) | t) : 0x7fffffff;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double v2 = __shfl_xor(v, o);
+            const int k2 = __shfl_xor(key, o);
+            if (v2 > v || (v2 == v && k2 < key)) { v = v2; key = k2; }
         }
+        const int b = k & 1;
+        if (lane == 0) { wv[b][w] = v; wk[b][w] = key; }
         __syncthreads();
-        if (which == 0) {               // forward: X[i][j] = -sum_{j <= k < i} L[i][k] X[k][j]
-            for (int i = 0; i < SB; ++i) {
-                T s = i == j ? one : zero;
-                if (i > j) {
-                    for (int k = j; k < i; ++k) s = tsub(s, tmul(M[i][k], X[k][j]));
-                }
-                X[i][j] = i < j ? zero : s;
+        v = wv[b][0];
+        key = wk[b][0];
+#pragma unroll
+        for (int q = 1; q < NW; ++q) {
+            const double v2 = wv[b][q];
+            const int k2 = wk[b][q];
+            if (v2 > v || (v2 == v && k2 < key)) { v = v2; key = k2; }
+        }
+        const int ps = key & 0xffff, p = key >> 16;
+        if (t == 0) order[k] = ps;
+        if (t == ps) {
+            live = false;
+        } else if (live) {
+            if (pos == k) pos = p;
+            const T pv = S[k * R + ps];
+            const T l = (pv.x == 0 && pv.y == 0) ? a : tdiv(a, pv);
+            S[k * R + t] = l;
+            for (int c = k + 1; c < nc; ++c) S[c * R + t] = tsub(S[c * R + t], tmul(l, S[c * R + ps]));
+        }
+    }
+    __syncthreads();
+    return npiv;
+}
+
+// The tournament of panel [j0, j0 + jb) over rows [j0, n). Leaves: blockIdx.x (P of them). The root writes the
+// panel's L11 \ U11 (top, column-major TW x TW), the interchanges (ipiv[j0 .. j0 + jb), LAPACK's convention) and
+// the row moves they make as (destination, source) pairs (pairs[0] = count), and info for a zero pivot.
+template <typename T>
+__global__ __launch_bounds__(Tour<T>::RL) void k_tslu(const T *A, long lda, long n, long j0, int jb, int P,
+                                                     TNode<T> *nodes, int *arrive, T *top, int *ipiv, int *pairs,
+                                                     int *info)
+{
+    constexpr int RL = Tour<T>::RL, AR = Tour<T>::AR, NT = RL, NW = NT / 64;
+    extern __shared__ char smem_raw[];
+    T *S = (T *)smem_raw;                   // RL x TW
+    __shared__ int ids[RL];
+    __shared__ int order[TW];
+    __shared__ double wv[2][NW];
+    __shared__ int wk[2][NW];
+    __shared__ int go;
+    const int t = threadIdx.x;
+    int idx = blockIdx.x;
+    const long r0 = j0 + (long)idx * RL;
+    int nr = (int)std::min((long)RL, n - r0);
+    for (int e = t; e < RL * jb; e += NT) {
+        const int c = e / RL, r = e % RL;
+        if (r < nr) S[c * RL + r] = A[(j0 + c) * lda + r0 + r];
+    }
+    if (t < nr) ids[t] = (int)(r0 + t);
+    __syncthreads();
+    int cnt = lds_gepp<T, NT>(S, RL, nr, jb, order, wv, wk);
+    int lsize = P, loff = 0;
+    while (lsize > 1) {
+        const int parent = idx / AR, first = parent * AR;
+        const int nchild = std::min(AR, lsize - first);
+        const int poff = loff + lsize;
+        if (nchild > 1) {
+            // nominate: this node's pivot rows, their original values from A (untouched during the launch)
+            TNode<T> *nd = nodes + loff + idx;
+            if (t == 0) nd->count = cnt;
+            if (t < cnt) nd->id[t] = ids[order[t]];
+            for (int e = t; e < cnt * jb; e += NT) {
+                const int c = e / cnt, i = e % cnt;
+                nd->v[c * TW + i] = A[(j0 + c) * lda + ids[order[i]]];
             }
-        } else {                        // backward: X[i][j] = (delta_ij - sum_{i < k <= j} U[i][k] X[k][j]) / U[i][i]
-            for (int i = SB - 1; i >= 0; --i) {
-                T s = i == j ? one : zero;
-                if (i <= j) {
-                    for (int k = i + 1; k <= j; ++k) s = tsub(s, tmul(M[i][k], X[k][j]));
-                    X[i][j] = tdiv(s, M[i][i]);
-                } else {
-                    X[i][j] = zero;
+            __syncthreads();
+            if (t == 0) {
+                __threadfence();
+                const int old = __hip_atomic_fetch_add(arrive + poff + parent, 1, __ATOMIC_ACQ_REL,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+                go = old == nchild - 1;
+                if (go) __hip_atomic_store(arrive + poff + parent, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __threadfence();
+            }
+            __syncthreads();
+            if (!go) return;
+            __threadfence();
+            // the last child to arrive factors the node: its children's nominees stacked in child order
+            nr = 0;
+            for (int ch = 0; ch < nchild; ++ch) {
+                const TNode<T> *cn = nodes + loff + first + ch;
+                const int cc = cn->count;
+                for (int e = t; e < cc * jb; e += NT) {
+                    const int c = e / cc, i = e % cc;
+                    S[c * RL + nr + i] = cn->v[c * TW + i];
                 }
+                if (t < cc) ids[nr + t] = cn->id[t];
+                nr += cc;
+            }
+            __syncthreads();
+            cnt = lds_gepp<T, NT>(S, RL, nr, jb, order, wv, wk);
+        }
+        // a node with one child nominates the child's rows unchanged
+        loff = poff;
+        lsize = (lsize + AR - 1) / AR;
+        idx = parent;
+    }
+    // the root: cnt == jb pivot rows
+    for (int e = t; e < TW * TW; e += NT) {
+        const int c = e / TW, i = e % TW;
+        top[e] = (c < jb && i < jb) ? S[c * RL + order[i]] : CX<T>::make(i == c, 0);
+    }
+    if (t == 0 && info) {
+        for (int k = 0; k < jb; ++k) {
+            const T d = S[k * RL + order[k]];
+            if (d.x == 0 && d.y == 0) {
+                if (*info == 0) *info = (int)(j0 + k) + 1;
+                break;
             }
         }
+    }
+    if (t < 64) {
+        // LAPACK's interchanges: winner k to position j0 + k in order. Rows below the panel's first jb move only
+        // when they win; lane i tracks loc (the position of the row first at j0 + i) and cont (the row at j0 + i).
+        const int i = t;
+        const int top_end = (int)j0 + jb;
+        int loc = (int)j0 + i, cont = (int)j0 + i;
+        const int win = i < jb ? ids[order[i]] : 0;
+        for (int k = 0; k < jb; ++k) {
+            const int w = __builtin_amdgcn_readlane(win, k);
+            const int tpos = (int)j0 + k;
+            const int pw = w < top_end ? __builtin_amdgcn_readlane(loc, w - (int)j0) : w;
+            if (i == 0) ipiv[tpos] = pw;
+            if (pw != tpos) {
+                const int x = __builtin_amdgcn_readlane(cont, k);        // a row first in the panel's top
+                if (i == k) cont = w;
+                if (w < top_end && i == w - (int)j0) loc = tpos;
+                if (pw < top_end && i == pw - (int)j0) cont = x;
+                if (i == x - (int)j0) loc = pw;
+            }
+        }
+        // the moves: position j0 + i takes row cont; a row of the top that ended below takes its place there
+        const bool mv_top = i < jb && cont != (int)j0 + i;
+        const bool mv_low = i < jb && loc >= top_end;
+        const unsigned long long m1 = __ballot(mv_top), m2 = __ballot(mv_low);
+        const unsigned long long below = (1ull << i) - 1;
+        const int n1 = __popcll(m1);
+        if (mv_top) {
+            const int q = __popcll(m1 & below);
+            pairs[1 + 2 * q] = (int)j0 + i;
+            pairs[2 + 2 * q] = cont;
+        }
+        if (mv_low) {
+            const int q = n1 + __popcll(m2 & below);
+            pairs[1 + 2 * q] = loc;
+            pairs[2 + 2 * q] = (int)j0 + i;
+        }
+        if (i == 0) pairs[0] = n1 + __popcll(m2);
+    }
+}
+
+// The inverses of the panel's L11 (block 0) and U11 (block 1), column-major 64 x 64, for L21 = A21 U11^-1 and
+// U12 = L11^-1 A12 on the matrix cores.
+template <typename T>
+__global__ __launch_bounds__(64) void k_trtri_top(const T *top, int jb, T *Linv, T *Uinv)
+{
+    __shared__ T M[SB][SB + 1];
+    __shared__ T X[SB][SB + 1];
+    const int which = blockIdx.x;
+    trtri64(top, TW, jb, which, M, X, which ? Uinv : Linv, true);
+}
+
+// The panel's row moves on columns [c0, c0 + nc): a column per 128 threads, every source read before any write;
+// the panel's own columns [j0, j0 + jb) then take the root's L11 \ U11 in rows [j0, j0 + jb).
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_moves(T *A, long lda, long c0, long nc, const int *pairs, long j0,
+                                                     int jb, const T *top)
+{
+    const int t = threadIdx.x & 127, h = threadIdx.x >> 7;
+    const int np = pairs[0];
+    const int dst = t < np ? pairs[1 + 2 * t] : 0, src = t < np ? pairs[2 + 2 * t] : 0;
+    for (long cb = (long)blockIdx.x * 8; cb < std::min(nc, (long)blockIdx.x * 8 + 8); cb += 2) {
+        const long c = c0 + cb + h;
+        const bool on = cb + h < nc;
+        T *col = A + c * lda;
+        T v = CX<T>::make(0, 0);
+        if (on && t < np) v = col[src];
         __syncthreads();
-        T *out = Dinv + ((size_t)blockIdx.x * 2 + which) * SB * SB;
-        for (int i = 0; i < SB; ++i) out[i * SB + j] = X[i][j];
+        if (on && t < np) col[dst] = v;
         __syncthreads();
+        if (on && c >= j0 && c < j0 + jb)
+            for (int i = t; i < jb; i += 128) col[j0 + i] = top[(c - j0) * TW + i];
     }
 }
 
@@ -619,10 +880,72 @@ int getrf_t(T *A, long n, long lda, int *ipiv, int *info, void *work, size_t wby
             if (int rc = check_launch("getrf/trsm")) return rc;
             const long M = n - c0;
             dim3 grid(blocks(M, GM), blocks(M, GN));
-            hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, s, A + c0 * lda + c0, lda, A + j0 * lda + c0, lda,
+            hipLaunchKernelGGL((k_gemm<T, true>), grid, dim3(256), 0, s, A + c0 * lda + c0, lda, A + j0 * lda + c0, lda,
                                A + c0 * lda + j0, lda, M, M, (long)jb);
             if (int rc = check_launch("getrf/gemm")) return rc;
         }
+    }
+    return M25X_OK;
+}
+
+template <typename T> size_t tour_work_bytes(long n)
+{
+    const long nn = tour_nodes(n, Tour<T>::RL, Tour<T>::AR);
+    return nn * sizeof(TNode<T>) + 3 * (size_t)TW * TW * sizeof(T) + sizeof(int) * ((size_t)nn + 4 * TW + 64);
+}
+
+template <typename T>
+int getrf_tp(T *A, long n, long lda, int *ipiv, int *info, void *work, size_t wbytes, hipStream_t s)
+{
+    constexpr int RL = Tour<T>::RL, AR = Tour<T>::AR;
+    if (wbytes < tour_work_bytes<T>(n))
+        return fail(M25X_EINVAL, "getrf: workspace of %zu bytes, %zu needed", wbytes, tour_work_bytes<T>(n));
+    const long nn = tour_nodes(n, RL, AR);
+    char *w = (char *)work;
+    TNode<T> *nodes = (TNode<T> *)w;
+    w += nn * sizeof(TNode<T>);
+    T *top = (T *)w, *Linv = top + TW * TW, *Uinv = Linv + TW * TW;
+    w += 3 * (size_t)TW * TW * sizeof(T);
+    int *arrive = (int *)w, *pairs = arrive + nn;
+    const size_t smem = (size_t)RL * TW * sizeof(T);
+    {
+        static bool attr_done[2] = {false, false};
+        static std::mutex mu;
+        std::lock_guard<std::mutex> lk(mu);
+        if (!attr_done[sizeof(T) == 8]) {
+            M25X_HIP(hipFuncSetAttribute((const void *)k_tslu<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+            attr_done[sizeof(T) == 8] = true;
+        }
+    }
+    hipLaunchKernelGGL(k_zero_ints, dim3(blocks(nn, 256)), dim3(256), 0, s, arrive, (int)nn);
+    if (info) hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, info, 1);
+    if (int rc = check_launch("getrf")) return rc;
+    for (long j0 = 0; j0 < n; j0 += TW) {
+        const int jb = (int)std::min((long)TW, n - j0);
+        const long m = n - j0;
+        const int P = (int)tour_leaves(m, RL);
+        hipLaunchKernelGGL(k_tslu<T>, dim3(P), dim3(RL), (unsigned)smem, s, (const T *)A, lda, n, j0, jb, P, nodes,
+                           arrive, top, ipiv, pairs, info);
+        hipLaunchKernelGGL(k_trtri_top<T>, dim3(2), dim3(64), 0, s, (const T *)top, jb, Linv, Uinv);
+        hipLaunchKernelGGL(k_apply_moves<T>, dim3(blocks(n, 8)), dim3(256), 0, s, A, lda, 0L, n, (const int *)pairs,
+                           j0, jb, (const T *)top);
+        if (int rc = check_launch("getrf/panel")) return rc;
+        if (m > jb) {   // L21 = A21 U11^-1
+            T *A21 = A + j0 * lda + j0 + jb;
+            hipLaunchKernelGGL((k_gemm<T, false>), dim3(blocks(m - jb, GM), 1), dim3(256), 0, s, A21, lda,
+                               (const T *)A21, lda, (const T *)Uinv, (long)TW, m - jb, (long)jb, (long)jb);
+        }
+        const long c0 = j0 + jb;
+        if (c0 < n) {   // U12 = L11^-1 A12, A22 -= L21 U12
+            T *A12 = A + c0 * lda + j0;
+            hipLaunchKernelGGL((k_gemm<T, false>), dim3(1, blocks(n - c0, GN)), dim3(256), 0, s, A12, lda,
+                               (const T *)Linv, (long)TW, (const T *)A12, lda, (long)jb, n - c0, (long)jb);
+            const long M = n - c0;
+            hipLaunchKernelGGL((k_gemm<T, true>), dim3(blocks(M, GM), blocks(M, GN)), dim3(256), 0, s,
+                               A + c0 * lda + c0, lda, (const T *)(A + j0 * lda + c0), lda, (const T *)A12, lda, M, M,
+                               (long)jb);
+        }
+        if (int rc = check_launch("getrf/update")) return rc;
     }
     return M25X_OK;
 }
@@ -686,17 +1009,25 @@ extern "C" {
 
 size_t m25x_getrf_work_bytes(int dtype, long n)
 {
-    return dtype == M25X_C64 ? work_bytes<float2>(n) : work_bytes<double2>(n);
+    return dtype == M25X_C64 ? std::max(work_bytes<float2>(n), tour_work_bytes<float2>(n))
+                             : std::max(work_bytes<double2>(n), tour_work_bytes<double2>(n));
 }
 
-int m25x_getrf(int dtype, void *A, long n, long lda, int *ipiv, int *info, void *work, size_t wbytes, void *stream)
+int m25x_getrf(int dtype, int pivot, void *A, long n, long lda, int *ipiv, int *info, void *work, size_t wbytes,
+               void *stream)
 {
     if (!A || !ipiv || !work || n < 0 || lda < n) return fail(M25X_EINVAL, "getrf: bad arguments");
+    if (pivot != M25X_PIVOT_PARTIAL && pivot != M25X_PIVOT_TOURNAMENT) return fail(M25X_EINVAL, "getrf: pivot %d", pivot);
     if (n == 0) return M25X_OK;
     if (n > (1L << 30)) return fail(M25X_EINVAL, "getrf: n too large");
     hipStream_t s = (hipStream_t)stream;
-    if (dtype == M25X_C128) return getrf_t<double2>((double2 *)A, n, lda, ipiv, info, work, wbytes, s);
-    if (dtype == M25X_C64) return getrf_t<float2>((float2 *)A, n, lda, ipiv, info, work, wbytes, s);
+    const bool tp = pivot == M25X_PIVOT_TOURNAMENT;
+    if (dtype == M25X_C128)
+        return tp ? getrf_tp<double2>((double2 *)A, n, lda, ipiv, info, work, wbytes, s)
+                  : getrf_t<double2>((double2 *)A, n, lda, ipiv, info, work, wbytes, s);
+    if (dtype == M25X_C64)
+        return tp ? getrf_tp<float2>((float2 *)A, n, lda, ipiv, info, work, wbytes, s)
+                  : getrf_t<float2>((float2 *)A, n, lda, ipiv, info, work, wbytes, s);
     return fail(M25X_EINVAL, "getrf: dtype %d", dtype);
 }
 

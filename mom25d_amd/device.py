@@ -280,8 +280,12 @@ class Factor:
         return self.LU.numel() * self.LU.element_size()
 
 
-def getrf(Z):
-    """LU of A = Z^T in place of the row-major square device tensor Z (complex128 or complex64)."""
+def getrf(Z, pivot='tournament'):
+    """LU of A = Z^T in place of the row-major square device tensor Z (complex128 or complex64). pivot:
+    'tournament' (CALU: a panel's pivots by a tree of GEPPs, no per-column grid barrier) or 'partial' (LAPACK's
+    pivots exactly)."""
+    if pivot not in ('tournament', 'partial'):
+        raise ValueError(f"getrf: pivot {pivot!r}")
     if Z.dim() != 2 or Z.shape[0] != Z.shape[1] or Z.stride(1) != 1 or Z.dtype not in (c128, c64):
         raise ValueError("getrf: a square row-major complex tensor")
     n, dv = Z.shape[0], device()
@@ -290,7 +294,8 @@ def getrf(Z):
     info = torch.zeros(1, dtype=i32, device=dv)
     wb = L.lib().m25x_getrf_work_bytes(dt, n)
     work = torch.empty(wb, dtype=torch.uint8, device=dv)
-    L.call('m25x_getrf', dt, ptr(Z), n, Z.stride(0), ptr(ipiv), ptr(info), ptr(work), wb, stream())
+    pv = L.PIVOT_TOURNAMENT if pivot == 'tournament' else L.PIVOT_PARTIAL
+    L.call('m25x_getrf', dt, pv, ptr(Z), n, Z.stride(0), ptr(ipiv), ptr(info), ptr(work), wb, stream())
     perm = torch.empty(n, dtype=i32, device=dv)
     L.call('m25x_ipiv_perm', ptr(ipiv), n, ptr(perm), stream())
     dinv = torch.empty(L.lib().m25x_dinv_bytes(dt, n), dtype=torch.uint8, device=dv)

@@ -1,6 +1,7 @@
 """The GPU LU and its solves against LAPACK (SciPy's lu_factor / lu_solve), the reference's factorisation
 (src/mom25d/fast.py:318-358) and its GPU-backend test (tests/test_gpu.py:210-235 of the reference): the same
-pivots, the factor in lu_factor's layout, and the solves of both transposes.
+pivots, the factor in lu_factor's layout, and the solves of both transposes. The tournament-pivoting path against
+its NumPy restatement (oracle/lu.py): the same interchanges, P A = L U, and the same solves.
 
 Tolerances: complex128 - backward error 1e-13 and the factor within 1e-12 of LAPACK's; complex64 - the
 reference's own bounds, |lu - ref| < 1e-4 max|ref| and solves within 1e-4 (test_gpu.py:231-235).
@@ -10,14 +11,15 @@ import pytest
 import scipy.linalg as sla
 import torch
 
+from oracle import lu as olu
 from tests.cases import seeded_matrix
 
 pytestmark = pytest.mark.gpu
 
 
-def _factor(dv, Z):
+def _factor(dv, Z, pivot='partial'):
     Zd = torch.from_numpy(np.ascontiguousarray(Z)).to(dv.device())
-    F = dv.getrf(Zd)
+    F = dv.getrf(Zd, pivot=pivot)
     torch.cuda.synchronize()
     return F
 
@@ -72,11 +74,12 @@ def test_getrf_selfcheck_matrix(gpu):
     A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))).astype(np.complex64) / np.sqrt(n)
     A[np.diag_indices(n)] += 3
     b = rng.standard_normal((n, 2)).astype(complex)
-    for dt, tol in ((np.complex64, 1e-4), (np.complex128, 1e-12)):
-        F = _factor(dv, A.astype(dt))
-        x = dv.getrs(F, torch.from_numpy(b).to(dv.device()), trans=1).cpu().numpy()
-        res = np.linalg.norm(A.astype(complex) @ x - b) / np.linalg.norm(b)
-        assert res < tol, (dt, res)
+    for pivot in ('partial', 'tournament'):
+        for dt, tol in ((np.complex64, 1e-4), (np.complex128, 1e-12)):
+            F = _factor(dv, A.astype(dt), pivot)
+            x = dv.getrs(F, torch.from_numpy(b).to(dv.device()), trans=1).cpu().numpy()
+            res = np.linalg.norm(A.astype(complex) @ x - b) / np.linalg.norm(b)
+            assert res < tol, (pivot, dt, res)
 
 
 def test_zgemm_nk(gpu):
@@ -94,5 +97,52 @@ def test_singular_pivot_reports_info(gpu):
     Z = seeded_matrix(100)
     Z[:, 5] = 0                                            # column 5 of Z = row 5 of A: zero from the start
     Z[5, :] = 0
-    F = _factor(dv, Z)
-    assert int(F.info.item()) == 6                        # LAPACK's info: 1 + the first zero pivot
+    for pivot in ('partial', 'tournament'):
+        F = _factor(dv, Z, pivot)
+        assert int(F.info.item()) == 6, pivot             # LAPACK's info: 1 + the first zero pivot
+
+
+def _check_tournament(dv, Z, F, RL, AR, tol_fac, tol_solve, full_ipiv):
+    n = Z.shape[0]
+    A = Z.T.astype(complex)
+    ipiv = F.ipiv.cpu().numpy()
+    assert np.all(ipiv >= np.arange(n)) and np.all(ipiv < n)
+    if full_ipiv:
+        LUo, ipo = olu.getrf_tournament(A, RL=RL, AR=AR)
+        assert np.array_equal(ipiv, ipo)
+        assert np.abs(F.LU.cpu().numpy().T - LUo).max() <= tol_fac * np.abs(LUo).max()
+    else:                                                  # the first panel's election, from the same inputs
+        win = olu.tslu_winners(A, 0, min(64, n), RL, AR)
+        assert np.array_equal(ipiv[:len(win)], olu.interchanges(win, 0, np.arange(n)))
+    lu = F.LU.cpu().numpy().T.astype(complex)
+    Lf, Uf = np.tril(lu, -1) + np.eye(n), np.triu(lu)
+    assert np.abs(A[olu.perm_of(ipiv)] - Lf @ Uf).max() <= tol_fac * np.abs(A).max()
+    rng = np.random.default_rng(n)
+    R = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    x = np.linalg.solve(Z.astype(complex), R)
+    X = dv.getrs(F, torch.from_numpy(R).to(dv.device()), trans=1).cpu().numpy()
+    assert np.abs(X - x).max() <= tol_solve * np.abs(x).max()
+    X0 = dv.getrs(F, torch.from_numpy(R).to(dv.device()), trans=0).cpu().numpy()
+    x0 = np.linalg.solve(A, R)
+    assert np.abs(X0 - x0).max() <= tol_solve * np.abs(x0).max()
+
+
+@pytest.mark.parametrize('n', [1, 7, 64, 65, 129, 300, 1000, 2000])
+def test_getrf_c128_tournament_matches_oracle(gpu, n):
+    """complex128 tournament pivoting (leaves of 128 rows, binary tree): the oracle's interchanges exactly, the
+    factor within 1e-11, solves within 1e-10."""
+    Z = seeded_matrix(n)
+    F = _factor(gpu, Z, 'tournament')
+    assert int(F.info.item()) == 0
+    _check_tournament(gpu, Z, F, 128, 2, 1e-11, 1e-10, True)
+
+
+@pytest.mark.parametrize('n', [300, 1000, 2000, 5000])
+def test_getrf_c64_tournament(gpu, n):
+    """complex64 tournament pivoting (leaves of 256 rows, 4 children a node; n = 5000 has nodes with a single
+    child): the first panel's election as the oracle's, P A = L U to FP32 accuracy, solves within the reference's
+    1e-4."""
+    Z = seeded_matrix(n, dtype=np.complex64)
+    F = _factor(gpu, Z, 'tournament')
+    assert int(F.info.item()) == 0
+    _check_tournament(gpu, Z, F, 256, 4, 5e-5, 1e-4, False)
