@@ -142,11 +142,12 @@ size_t m25x_getrf_work_bytes(int dtype, long n);
 /* LU of the column-major n x n matrix A (leading dimension lda; the row-major Z seen as Z^T), in place: L unit
  * lower, U upper, ipiv 0-based LAPACK-style interchanges (int32, device), the layout of
  * scipy.linalg.lu_factor(Z.T). complex128 (trailing updates on FP64 MFMA) or complex64 (FP32 MFMA, FP32
- * accumulation). pivot: M25X_PIVOT_PARTIAL or M25X_PIVOT_TOURNAMENT. info (device int, may be NULL): 0, or 1 + the
- * first zero pivot. Replaces the GPU backend's factor (src/mom25d/gpu.py:69-74; src/mom25d/gpu_vk.py:515-633) and
- * LAPACK zgetrf/cgetrf (src/mom25d/fast.py:318-320). */
-int m25x_getrf(int dtype, int pivot, void *A, long n, long lda, int *ipiv, int *info, void *work, size_t work_bytes,
-               void *stream);
+ * accumulation). pivot: M25X_PIVOT_PARTIAL or M25X_PIVOT_TOURNAMENT. perm (device int[n], may be NULL): the
+ * interchanges as one permutation, m25x_ipiv_perm's. info (device int, may be NULL): 0, or 1 + the first zero
+ * pivot. Replaces the GPU backend's factor (src/mom25d/gpu.py:69-74; src/mom25d/gpu_vk.py:515-633) and LAPACK
+ * zgetrf/cgetrf (src/mom25d/fast.py:318-320). */
+int m25x_getrf(int dtype, int pivot, void *A, long n, long lda, int *ipiv, int *perm, int *info, void *work,
+               size_t work_bytes, void *stream);
 
 /* The interchanges of ipiv as one permutation (perm = ipiv's swaps applied in order to 0..n-1), once per factor:
  * P^T b = b[perm], P z scatters z[i] to perm[i]. */
@@ -155,10 +156,12 @@ int m25x_ipiv_perm(const int *ipiv, long n, int *perm, void *stream);
 /* Device workspace bytes m25x_getrs needs. */
 size_t m25x_getrs_work_bytes(int dtype, long n, long ldb);
 
-/* Device bytes of the inverted diagonal blocks of a factor (two 64 x 64 blocks per 64 rows). */
+/* Device bytes of a factor's block solve operators (six 64 x 64 blocks per 64 rows). */
 size_t m25x_dinv_bytes(int dtype, long n);
 
-/* The inverses of the factor's 64 x 64 diagonal blocks (L_bb^-1 and U_bb^-1), once per factor, for m25x_getrs. */
+/* A factor's block solve operators, once per factor, for m25x_getrs: the inverses of its 64 x 64 diagonal blocks
+ * (L_bb^-1, U_bb^-1) and, for each of the four triangular solves, the product of a block's diagonal inverse with
+ * the off-diagonal block that couples it to the block solved before it. */
 int m25x_trtri_blocks(int dtype, const void *LU, long n, long lda, void *dinv, void *stream);
 
 /* Solves with the factor of m25x_getrf: trans = 1: Z X = B (A^T X = B, lu_solve(..., trans=1)), trans = 0:

@@ -46,7 +46,7 @@ _SIGS = {
     'm25x_scale_c64': ([P, L, L, P, P, L, P], I),
     'm25x_diag_scales': ([P, L, L, P, P], I),
     'm25x_getrf_work_bytes': ([I, L], S),
-    'm25x_getrf': ([I, I, P, L, L, P, P, P, S, P], I),
+    'm25x_getrf': ([I, I, P, L, L, P, P, P, P, S, P], I),
     'm25x_ipiv_perm': ([P, L, P, P], I),
     'm25x_getrs_work_bytes': ([I, L, L], S),
     'm25x_dinv_bytes': ([I, L], S),

@@ -16,65 +16,15 @@
 // grid barrier (k_tslu), its row moves on every column (k_apply_moves), then L21 = A21 U11^-1, U12 = L11^-1 A12 and
 // A22 -= L21 U12, all three on the matrix cores.
 // The solves (getrs) run one workgroup per 64-row block with ready flags between blocks (no launch per block), each
-// block's diagonal solve a product with its inverse (k_trtri_diag, once per factor).
-#include <algorithm>
+// block's diagonal solve a product with its inverse and the step from its predecessor one precomputed 64 x 64
+// product (k_trtri_diag and k_solve_ops, once per factor).
 #include <mutex>
-#include "m25x_common.h"
+#include "m25x_lu_dev.h"
 
 using namespace m25x;
+using namespace m25x::lu;
 
 namespace {
-
-typedef double d4v __attribute__((ext_vector_type(4)));
-typedef float f4v __attribute__((ext_vector_type(4)));
-
-// ---- complex traits ----------------------------------------------------------------------------------------
-template <typename T> struct CX;
-template <> struct CX<double2> {
-    typedef double R;
-    typedef d4v acc4;
-    __device__ static double2 make(double x, double y) { return make_double2(x, y); }
-    __device__ static acc4 mfma(double a, double b, acc4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-    // the output row of lane group kk (lane / 16), register r (measured on gfx950: kk + 4 r for the f64 form)
-    __device__ static int out_row(int kk, int r) { return kk + 4 * r; }
-};
-template <> struct CX<float2> {
-    typedef float R;
-    typedef f4v acc4;
-    __device__ static float2 make(float x, float y) { return make_float2(x, y); }
-    __device__ static acc4 mfma(float a, float b, acc4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-    // (4 kk + r for the f32 form)
-    __device__ static int out_row(int kk, int r) { return 4 * kk + r; }
-};
-
-template <typename T> __device__ inline T tadd(T a, T b) { return CX<T>::make(a.x + b.x, a.y + b.y); }
-template <typename T> __device__ inline T tsub(T a, T b) { return CX<T>::make(a.x - b.x, a.y - b.y); }
-template <typename T> __device__ inline T tmul(T a, T b)
-{
-    return CX<T>::make(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-// reciprocal by Smith's algorithm
-template <typename T> __device__ inline T trecip(T b)
-{
-    typedef typename CX<T>::R R;
-    if (fabs((double)b.x) >= fabs((double)b.y)) {
-        R r = b.y / b.x, den = b.x + b.y * r;
-        return CX<T>::make((R)1 / den, -r / den);
-    }
-    R r = b.x / b.y, den = b.x * r + b.y;
-    return CX<T>::make(r / den, (R)-1 / den);
-}
-template <typename T> __device__ inline T tdiv(T a, T b)
-{
-    typedef typename CX<T>::R R;
-    if (fabs((double)b.x) >= fabs((double)b.y)) {
-        R r = b.y / b.x, den = b.x + b.y * r;
-        return CX<T>::make((a.x + a.y * r) / den, (a.y - a.x * r) / den);
-    }
-    R r = b.x / b.y, den = b.x * r + b.y;
-    return CX<T>::make((a.x * r + a.y) / den, (a.y * r - a.x) / den);
-}
-template <typename T> __device__ inline double cabs1(T a) { return fabs((double)a.x) + fabs((double)a.y); }
 
 // ---- grid barrier (all workgroups of a cooperative launch co-resident) -------------------------------------
 struct Bar {
@@ -115,6 +65,12 @@ __global__ void k_zero_ints(int *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+__global__ void k_iota(int *p, long n)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int)i;
 }
 
 // ---- panel --------------------------------------------------------------------------------------------------
@@ -417,6 +373,9 @@ __global__ void k_perm(const int *ipiv, long n, int *perm)
 // block (inverted once per factor by k_trtri_diag): no serial substitution on the critical path.
 constexpr int SB = 64;
 constexpr int MAXRHS = 8;
+// Per 64-row block b, NSLOT 64 x 64 row-major operators: 0 L_bb^-1, 1 U_bb^-1, 2 + kind the solve's step matrix
+// G_b = Mi_b Op_{b, pred(b)} of each kind (Mi_b the kind's diagonal inverse, pred(b) the block published before b).
+constexpr int NSLOT = 6;
 
 // The inverse of the unit lower (which 0) or upper (which 1) triangle of the bn x bn block at LU (column-major,
 // leading dimension lda), padded to 64 x 64 with the identity; out[i * SB + j] (row-major) or out[j * SB + i]
@@ -439,24 +398,25 @@ __device__ void trtri64(const T *LU, long lda, int bn, int which, T (*M)[SB + 1]
         M[i][j] = v;
     }
     __syncthreads();
+    // sum_{k0 <= k < k1} M[i][k] X[k][j] with four independent partial sums (the loads of a group issued together)
+    auto dot = [&](int i, int k0, int k1) {
+        T s0 = zero, s1 = zero, s2 = zero, s3 = zero;
+        int k = k0;
+        for (; k + 4 <= k1; k += 4) {
+            s0 = tadd(s0, tmul(M[i][k], X[k][j]));
+            s1 = tadd(s1, tmul(M[i][k + 1], X[k + 1][j]));
+            s2 = tadd(s2, tmul(M[i][k + 2], X[k + 2][j]));
+            s3 = tadd(s3, tmul(M[i][k + 3], X[k + 3][j]));
+        }
+        for (; k < k1; ++k) s0 = tadd(s0, tmul(M[i][k], X[k][j]));
+        return tadd(tadd(s0, s1), tadd(s2, s3));
+    };
     if (which == 0) {               // forward: X[i][j] = -sum_{j <= k < i} L[i][k] X[k][j]
-        for (int i = 0; i < SB; ++i) {
-            T s = i == j ? one : zero;
-            if (i > j) {
-                for (int k = j; k < i; ++k) s = tsub(s, tmul(M[i][k], X[k][j]));
-            }
-            X[i][j] = i < j ? zero : s;
-        }
+        for (int i = 0; i < SB; ++i)
+            X[i][j] = i < j ? zero : (i == j ? one : tsub(zero, dot(i, j, i)));
     } else {                        // backward: X[i][j] = (delta_ij - sum_{i < k <= j} U[i][k] X[k][j]) / U[i][i]
-        for (int i = SB - 1; i >= 0; --i) {
-            T s = i == j ? one : zero;
-            if (i <= j) {
-                for (int k = i + 1; k <= j; ++k) s = tsub(s, tmul(M[i][k], X[k][j]));
-                X[i][j] = tdiv(s, M[i][i]);
-            } else {
-                X[i][j] = zero;
-            }
-        }
+        for (int i = SB - 1; i >= 0; --i)
+            X[i][j] = i > j ? zero : tdiv(tsub(i == j ? one : zero, dot(i, i + 1, j + 1)), M[i][i]);
     }
     __syncthreads();
     for (int i = 0; i < SB; ++i) {
@@ -476,213 +436,7 @@ __global__ __launch_bounds__(64) void k_trtri_diag(const T *LU, long lda, long n
     const long i0 = (long)blockIdx.x * SB;
     const int bn = (int)std::min((long)SB, n - i0);
     for (int which = 0; which < 2; ++which)
-        trtri64(LU + i0 * lda + i0, lda, bn, which, M, X, Dinv + ((size_t)blockIdx.x * 2 + which) * SB * SB, false);
-}
-
-// ---- tournament pivoting (communication-avoiding LU panels) -----------------------------------------------
-// A panel's TW pivot rows are chosen by a tree of GEPPs (Grigori, Demmel and Xiang, "CALU: a communication
-// optimal LU factorization algorithm", SIAM J. Matrix Anal. Appl. 32 (2011)): each leaf workgroup factors RL rows
-// of the panel in LDS and nominates its pivot rows; a tree node factors its children's nominees (their original
-// rows, stacked in child order) and nominates its own; the root's pivot rows are the panel's, and the root's
-// factors of them are the panel's L11 \ U11. One launch a panel: a node is factored by whichever of its children
-// arrives last (an atomic count per node), so no workgroup ever waits on another. The rows below take
-// L21 = A21 U11^-1 and the trailing matrix the usual update, on the matrix cores.
-constexpr int TW = 64;
-template <typename T> struct Tour;
-template <> struct Tour<double2> { static constexpr int RL = 128, AR = 2; };   // rows a leaf, children a node
-template <> struct Tour<float2> { static constexpr int RL = 256, AR = 4; };
-
-template <typename T> struct TNode {
-    int id[TW];                 // the nominees' global rows, in their pivot order
-    int count, pad[63];
-    T v[TW * TW];               // their original panel rows, column-major: v[c * TW + i]
-};
-
-inline long tour_leaves(long m, int RL) { return (m + RL - 1) / RL; }
-inline long tour_nodes(long n, int RL, int AR)
-{
-    long P = tour_leaves(n, RL), tot = 0;
-    while (P > 1) { tot += P; P = (P + AR - 1) / AR; }
-    return tot + 1;
-}
-
-// GEPP of the rows in slots [0, nr) of S (column-major, S[c * R + slot], the slots in the rows' stacking order)
-// over columns [0, nc); thread t owns slot t and rows never move: a row's position in the elimination order is
-// tracked instead, and ties in the pivot search go to the earliest position (LAPACK's rule). order[k] = the slot
-// of the k-th pivot. On return a slot chosen at step k holds row k of L \ U, the other slots their multipliers
-// and Schur complement. One barrier a step: the per-wave maxima are double-buffered.
-template <typename T, int NT>
-__device__ int lds_gepp(T *S, int R, int nr, int nc, int *order, double (*wv)[NT / 64], int (*wk)[NT / 64])
-{
-    constexpr int NW = NT / 64;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int npiv = nr < nc ? nr : nc;
-    const T zero = CX<T>::make(0, 0);
-    bool live = t < nr;
-    int pos = t;
-    for (int k = 0; k < npiv; ++k) {
-        const T a = live ? S[k * R + t] : zero;
-        double v = live ? cabs1(a) : -1.0;
-        if (live && !(v >= 0)) v = 0;        // a NaN never outranks a live row, and a live row always wins over none
-        int key = live ? ((pos << 16) | t) : 0x7fffffff;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const double v2 = __shfl_xor(v, o);
-            const int k2 = __shfl_xor(key, o);
-            if (v2 > v || (v2 == v && k2 < key)) { v = v2; key = k2; }
-        }
-        const int b = k & 1;
-        if (lane == 0) { wv[b][w] = v; wk[b][w] = key; }
-        __syncthreads();
-        v = wv[b][0];
-        key = wk[b][0];
-#pragma unroll
-        for (int q = 1; q < NW; ++q) {
-            const double v2 = wv[b][q];
-            const int k2 = wk[b][q];
-            if (v2 > v || (v2 == v && k2 < key)) { v = v2; key = k2; }
-        }
-        const int ps = key & 0xffff, p = key >> 16;
-        if (t == 0) order[k] = ps;
-        if (t == ps) {
-            live = false;
-        } else if (live) {
-            if (pos == k) pos = p;
-            const T pv = S[k * R + ps];
-            const T l = (pv.x == 0 && pv.y == 0) ? a : tdiv(a, pv);
-            S[k * R + t] = l;
-            for (int c = k + 1; c < nc; ++c) S[c * R + t] = tsub(S[c * R + t], tmul(l, S[c * R + ps]));
-        }
-    }
-    __syncthreads();
-    return npiv;
-}
-
-// The tournament of panel [j0, j0 + jb) over rows [j0, n). Leaves: blockIdx.x (P of them). The root writes the
-// panel's L11 \ U11 (top, column-major TW x TW), the interchanges (ipiv[j0 .. j0 + jb), LAPACK's convention) and
-// the row moves they make as (destination, source) pairs (pairs[0] = count), and info for a zero pivot.
-template <typename T>
-__global__ __launch_bounds__(Tour<T>::RL) void k_tslu(const T *A, long lda, long n, long j0, int jb, int P,
-                                                     TNode<T> *nodes, int *arrive, T *top, int *ipiv, int *pairs,
-                                                     int *info)
-{
-    constexpr int RL = Tour<T>::RL, AR = Tour<T>::AR, NT = RL, NW = NT / 64;
-    extern __shared__ char smem_raw[];
-    T *S = (T *)smem_raw;                   // RL x TW
-    __shared__ int ids[RL];
-    __shared__ int order[TW];
-    __shared__ double wv[2][NW];
-    __shared__ int wk[2][NW];
-    __shared__ int go;
-    const int t = threadIdx.x;
-    int idx = blockIdx.x;
-    const long r0 = j0 + (long)idx * RL;
-    int nr = (int)std::min((long)RL, n - r0);
-    for (int e = t; e < RL * jb; e += NT) {
-        const int c = e / RL, r = e % RL;
-        if (r < nr) S[c * RL + r] = A[(j0 + c) * lda + r0 + r];
-    }
-    if (t < nr) ids[t] = (int)(r0 + t);
-    __syncthreads();
-    int cnt = lds_gepp<T, NT>(S, RL, nr, jb, order, wv, wk);
-    int lsize = P, loff = 0;
-    while (lsize > 1) {
-        const int parent = idx / AR, first = parent * AR;
-        const int nchild = std::min(AR, lsize - first);
-        const int poff = loff + lsize;
-        if (nchild > 1) {
-            // nominate: this node's pivot rows, their original values from A (untouched during the launch)
-            TNode<T> *nd = nodes + loff + idx;
-            if (t == 0) nd->count = cnt;
-            if (t < cnt) nd->id[t] = ids[order[t]];
-            for (int e = t; e < cnt * jb; e += NT) {
-                const int c = e / cnt, i = e % cnt;
-                nd->v[c * TW + i] = A[(j0 + c) * lda + ids[order[i]]];
-            }
-            __syncthreads();
-            if (t == 0) {
-                __threadfence();
-                const int old = __hip_atomic_fetch_add(arrive + poff + parent, 1, __ATOMIC_ACQ_REL,
-                                                       __HIP_MEMORY_SCOPE_AGENT);
-                go = old == nchild - 1;
-                if (go) __hip_atomic_store(arrive + poff + parent, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __threadfence();
-            }
-            __syncthreads();
-            if (!go) return;
-            __threadfence();
-            // the last child to arrive factors the node: its children's nominees stacked in child order
-            nr = 0;
-            for (int ch = 0; ch < nchild; ++ch) {
-                const TNode<T> *cn = nodes + loff + first + ch;
-                const int cc = cn->count;
-                for (int e = t; e < cc * jb; e += NT) {
-                    const int c = e / cc, i = e % cc;
-                    S[c * RL + nr + i] = cn->v[c * TW + i];
-                }
-                if (t < cc) ids[nr + t] = cn->id[t];
-                nr += cc;
-            }
-            __syncthreads();
-            cnt = lds_gepp<T, NT>(S, RL, nr, jb, order, wv, wk);
-        }
-        // a node with one child nominates the child's rows unchanged
-        loff = poff;
-        lsize = (lsize + AR - 1) / AR;
-        idx = parent;
-    }
-    // the root: cnt == jb pivot rows
-    for (int e = t; e < TW * TW; e += NT) {
-        const int c = e / TW, i = e % TW;
-        top[e] = (c < jb && i < jb) ? S[c * RL + order[i]] : CX<T>::make(i == c, 0);
-    }
-    if (t == 0 && info) {
-        for (int k = 0; k < jb; ++k) {
-            const T d = S[k * RL + order[k]];
-            if (d.x == 0 && d.y == 0) {
-                if (*info == 0) *info = (int)(j0 + k) + 1;
-                break;
-            }
-        }
-    }
-    if (t < 64) {
-        // LAPACK's interchanges: winner k to position j0 + k in order. Rows below the panel's first jb move only
-        // when they win; lane i tracks loc (the position of the row first at j0 + i) and cont (the row at j0 + i).
-        const int i = t;
-        const int top_end = (int)j0 + jb;
-        int loc = (int)j0 + i, cont = (int)j0 + i;
-        const int win = i < jb ? ids[order[i]] : 0;
-        for (int k = 0; k < jb; ++k) {
-            const int w = __builtin_amdgcn_readlane(win, k);
-            const int tpos = (int)j0 + k;
-            const int pw = w < top_end ? __builtin_amdgcn_readlane(loc, w - (int)j0) : w;
-            if (i == 0) ipiv[tpos] = pw;
-            if (pw != tpos) {
-                const int x = __builtin_amdgcn_readlane(cont, k);        // a row first in the panel's top
-                if (i == k) cont = w;
-                if (w < top_end && i == w - (int)j0) loc = tpos;
-                if (pw < top_end && i == pw - (int)j0) cont = x;
-                if (i == x - (int)j0) loc = pw;
-            }
-        }
-        // the moves: position j0 + i takes row cont; a row of the top that ended below takes its place there
-        const bool mv_top = i < jb && cont != (int)j0 + i;
-        const bool mv_low = i < jb && loc >= top_end;
-        const unsigned long long m1 = __ballot(mv_top), m2 = __ballot(mv_low);
-        const unsigned long long below = (1ull << i) - 1;
-        const int n1 = __popcll(m1);
-        if (mv_top) {
-            const int q = __popcll(m1 & below);
-            pairs[1 + 2 * q] = (int)j0 + i;
-            pairs[2 + 2 * q] = cont;
-        }
-        if (mv_low) {
-            const int q = n1 + __popcll(m2 & below);
-            pairs[1 + 2 * q] = loc;
-            pairs[2 + 2 * q] = (int)j0 + i;
-        }
-        if (i == 0) pairs[0] = n1 + __popcll(m2);
-    }
+        trtri64(LU + i0 * lda + i0, lda, bn, which, M, X, Dinv + ((size_t)blockIdx.x * NSLOT + which) * SB * SB, false);
 }
 
 // The inverses of the panel's L11 (block 0) and U11 (block 1), column-major 64 x 64, for L21 = A21 U11^-1 and
@@ -697,14 +451,22 @@ __global__ __launch_bounds__(64) void k_trtri_top(const T *top, int jb, T *Linv,
 }
 
 // The panel's row moves on columns [c0, c0 + nc): a column per 128 threads, every source read before any write;
-// the panel's own columns [j0, j0 + jb) then take the root's L11 \ U11 in rows [j0, j0 + jb).
+// the panel's own columns [j0, j0 + jb) then take the root's L11 \ U11 in rows [j0, j0 + jb). With perm, one more
+// workgroup moves the entries of the row permutation (position -> original row) the same way.
 template <typename T>
 __global__ __launch_bounds__(256) void k_apply_moves(T *A, long lda, long c0, long nc, const int *pairs, long j0,
-                                                     int jb, const T *top)
+                                                     int jb, const T *top, int *perm)
 {
     const int t = threadIdx.x & 127, h = threadIdx.x >> 7;
     const int np = pairs[0];
     const int dst = t < np ? pairs[1 + 2 * t] : 0, src = t < np ? pairs[2 + 2 * t] : 0;
+    if (perm && blockIdx.x == gridDim.x - 1) {      // the extra workgroup: the same moves on the row permutation
+        int v = 0;
+        if (h == 0 && t < np) v = perm[src];
+        __syncthreads();
+        if (h == 0 && t < np) perm[dst] = v;
+        return;
+    }
     for (long cb = (long)blockIdx.x * 8; cb < std::min(nc, (long)blockIdx.x * 8 + 8); cb += 2) {
         const long c = c0 + cb + h;
         const bool on = cb + h < nc;
@@ -721,89 +483,188 @@ __global__ __launch_bounds__(256) void k_apply_moves(T *A, long lda, long c0, lo
 
 // kind 0: U^T y = b (forward); 1: L^T z = y (backward); 2: L y = b (forward); 3: U x = y (backward).
 // Operator entry (row i, column k): kinds 0, 1 read LU[i * lda + k] (U[k][i], L[k][i]); kinds 2, 3 LU[k * lda + i].
+// Diagonal inverse Mi: kind 0 (U_bb^-1)^T, 1 (L_bb^-1)^T, 2 L_bb^-1, 3 U_bb^-1.
+__device__ inline bool kind_fwd(int kind) { return kind == 0 || kind == 2; }
+
+// The step matrices G_b = Mi_b Op_{b, pred(b)} (grid: blocks x 4 kinds), after k_trtri_diag.
+template <typename T>
+__global__ __launch_bounds__(256) void k_solve_ops(const T *LU, long lda, long n, T *Dinv)
+{
+    __shared__ T Ms[SB][SB + 1];
+    __shared__ T Os[SB][SB + 1];
+    const int nb = (int)((n + SB - 1) / SB);
+    const int b = blockIdx.x, kind = blockIdx.y, tid = threadIdx.x;
+    const int p = kind_fwd(kind) ? b - 1 : b + 1;
+    T *G = Dinv + ((size_t)b * NSLOT + 2 + kind) * SB * SB;
+    const T zero = CX<T>::make(0, 0);
+    if (p < 0 || p >= nb) {
+        for (int idx = tid; idx < SB * SB; idx += 256) G[idx] = zero;
+        return;
+    }
+    const long i0 = (long)b * SB, k0 = (long)p * SB;
+    const int bn = (int)std::min((long)SB, n - i0), pn = (int)std::min((long)SB, n - k0);
+    const T *D = Dinv + ((size_t)b * NSLOT + ((kind == 0 || kind == 3) ? 1 : 0)) * SB * SB;
+    const bool tr = kind <= 1;
+    for (int idx = tid; idx < SB * SB; idx += 256) {
+        const int i = idx / SB, k = idx % SB;
+        Ms[i][k] = tr ? D[k * SB + i] : D[i * SB + k];
+        Os[i][k] = (i < bn && k < pn) ? (tr ? LU[(i0 + i) * lda + k0 + k] : LU[(k0 + k) * lda + i0 + i]) : zero;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < SB * SB; idx += 256) {
+        const int i = idx / SB, j = idx % SB;
+        T g0 = zero, g1 = zero, g2 = zero, g3 = zero;
+        for (int k = 0; k < SB; k += 4) {
+            g0 = tadd(g0, tmul(Ms[i][k], Os[k][j]));
+            g1 = tadd(g1, tmul(Ms[i][k + 1], Os[k + 1][j]));
+            g2 = tadd(g2, tmul(Ms[i][k + 2], Os[k + 2][j]));
+            g3 = tadd(g3, tmul(Ms[i][k + 3], Os[k + 3][j]));
+        }
+        G[idx] = tadd(tadd(g0, g1), tadd(g2, g3));
+    }
+}
+
+// Workgroup b (in publication order) owns a 64-row block: y_b = Mi_b (b_b - sum_{a before pred} Op_ba y_a)
+// - G_b y_pred. The sum is taken block by block as the flags come up (thread (row, slice) keeps partial sums over
+// its 16-column slice), and Mi_b applied, before the predecessor publishes; after its flag the critical path is one
+// 64 x 64 product with G_b (held in registers), a reduction and the publish.
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsv(const T *LU, long lda, long n, const T *Dinv, T *B, long ldb, int nrhs,
                                               int kind, int *flags, int *abort)
 {
-    __shared__ T Mi[SB][SB + 1];             // the diagonal block's inverse for this operator, Mi[i][k]
     __shared__ T acc[SB][MAXRHS];
+    __shared__ T ys[SB][MAXRHS];
     __shared__ T part[4][SB][MAXRHS];
+    __shared__ int s_avail;
     const int nb = (int)((n + SB - 1) / SB);
-    const bool fwd = kind == 0 || kind == 2;
+    const bool fwd = kind_fwd(kind);
     const int b = fwd ? blockIdx.x : nb - 1 - blockIdx.x;     // publication order is blockIdx order
     const long i0 = (long)b * SB;
     const int bn = (int)std::min((long)SB, n - i0);
     const int tid = threadIdx.x;
     const bool rowmajor = kind <= 1;
+    const T zero = CX<T>::make(0, 0);
+    const int row = tid & 63, kq = tid >> 6;                  // 4 slices of 16 columns
+    T g[16];
+    const T *D = Dinv + ((size_t)b * NSLOT + ((kind == 0 || kind == 3) ? 1 : 0)) * SB * SB;
     {
-        // kind 0: (U^-1)^T, 1: (L^-1)^T, 2: L^-1, 3: U^-1
-        const T *D = Dinv + ((size_t)b * 2 + ((kind == 0 || kind == 3) ? 1 : 0)) * SB * SB;
-        const bool tr = kind <= 1;
-        for (int idx = tid; idx < SB * SB; idx += 256) {
-            const int i = idx / SB, k = idx % SB;
-            Mi[i][k] = tr ? D[k * SB + i] : D[i * SB + k];
-        }
+        const T *Gb = Dinv + ((size_t)b * NSLOT + 2 + kind) * SB * SB;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) g[u] = Gb[row * SB + kq * 16 + u];
     }
     for (int idx = tid; idx < SB * MAXRHS; idx += 256) {
         const int i = idx / MAXRHS, r = idx % MAXRHS;
-        acc[i][r] = (i < bn && r < nrhs) ? B[(i0 + i) * ldb + r] : CX<T>::make(0, 0);
+        acc[i][r] = (i < bn && r < nrhs) ? B[(i0 + i) * ldb + r] : zero;
     }
-    __syncthreads();
-    const int row = tid & 63, kq = tid >> 6;                  // 4 slices of 16 columns
-    for (int q = 0; q < (int)blockIdx.x; ++q) {
-        const int a = fwd ? q : nb - 1 - q;
+    auto order = [&](int q) { return fwd ? q : nb - 1 - q; };
+    auto stage_y = [&](int a) {              // ys = y_a, zero padded
         const long k0 = (long)a * SB;
         const int kn = (int)std::min((long)SB, n - k0);
-        // this thread's 16 operator entries of block (b, a), loaded before the wait
-        T m[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int kx = kq * 16 + u;
-            const bool in = row < bn && kx < kn;
-            const long gi = i0 + row, gk = k0 + kx;
-            m[u] = in ? (rowmajor ? LU[gi * lda + gk] : LU[gk * lda + gi]) : CX<T>::make(0, 0);
+        for (int idx = tid; idx < SB * MAXRHS; idx += 256) {
+            const int i = idx / MAXRHS, r = idx % MAXRHS;
+            ys[i][r] = (i < kn && r < nrhs) ? B[(k0 + i) * ldb + r] : zero;
         }
+    };
+    auto wait_flag = [&](int a) {
+        long spins = 0;
+        while (__hip_atomic_load(flags + a, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > (1L << 26)) { __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
+        }
+    };
+    T s[MAXRHS];
+#pragma unroll
+    for (int r = 0; r < MAXRHS; ++r) s[r] = zero;
+    const int nprev = (int)blockIdx.x;
+    const int nbulk = nprev > 0 ? nprev - 1 : 0;
+    int done = 0;
+    while (done < nbulk) {
         if (tid == 0) {
-            long spins = 0;
-            while (__hip_atomic_load(flags + a, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > (1L << 26)) { __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
-            }
+            wait_flag(order(done));
+            int q = done + 1;
+            while (q < nbulk && __hip_atomic_load(flags + order(q), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0) ++q;
+            s_avail = q;
         }
         __syncthreads();
-        T s[MAXRHS];
+        const int avail = s_avail;
+        for (int q = done; q < avail; ++q) {
+            const int a = order(q);
+            const long k0 = (long)a * SB;
+            T m[16];
 #pragma unroll
-        for (int r = 0; r < MAXRHS; ++r) s[r] = CX<T>::make(0, 0);
+            for (int u = 0; u < 16; ++u) {
+                const long gi = i0 + row, gk = k0 + kq * 16 + u;
+                m[u] = (row < bn && gk < n) ? (rowmajor ? LU[gi * lda + gk] : LU[gk * lda + gi]) : zero;
+            }
+            stage_y(a);
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+#pragma unroll
+                for (int r = 0; r < MAXRHS; ++r) s[r] = tadd(s[r], tmul(m[u], ys[kq * 16 + u][r]));
+            __syncthreads();
+        }
+        done = avail;
+    }
+    // acc <- Mi (acc - sum)
+#pragma unroll
+    for (int r = 0; r < MAXRHS; ++r) part[kq][row][r] = s[r];
+    __syncthreads();
+    for (int idx = tid; idx < SB * MAXRHS; idx += 256) {
+        const int i = idx / MAXRHS, r = idx % MAXRHS;
+        acc[i][r] = tsub(acc[i][r], tadd(tadd(part[0][i][r], part[1][i][r]), tadd(part[2][i][r], part[3][i][r])));
+    }
+    __syncthreads();
+    {
+        T mi[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
-            const int kx = kq * 16 + u;
-            if (kx < kn) {
-                const T *y = B + (k0 + kx) * ldb;
-#pragma unroll
-                for (int r = 0; r < MAXRHS; ++r)
-                    if (r < nrhs) s[r] = tadd(s[r], tmul(m[u], y[r]));
-            }
+            const int k = kq * 16 + u;
+            mi[u] = rowmajor ? D[k * SB + row] : D[row * SB + k];
         }
 #pragma unroll
-        for (int r = 0; r < MAXRHS; ++r) part[kq][row][r] = s[r];
-        __syncthreads();
-        for (int idx = tid; idx < SB * nrhs; idx += 256) {
-            const int i = idx / nrhs, r = idx % nrhs;
-            acc[i][r] = tsub(acc[i][r], tadd(tadd(part[0][i][r], part[1][i][r]), tadd(part[2][i][r], part[3][i][r])));
+        for (int r = 0; r < MAXRHS; ++r) {
+            T v = zero;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) v = tadd(v, tmul(mi[u], acc[kq * 16 + u][r]));
+            part[kq][row][r] = v;
         }
-        __syncthreads();
     }
-    // the diagonal block: y = Mi acc
-    for (int idx = tid; idx < bn * nrhs; idx += 256) {
-        const int i = idx / nrhs, r = idx % nrhs;
-        T v = CX<T>::make(0, 0);
-        for (int k = 0; k < bn; ++k) v = tadd(v, tmul(Mi[i][k], acc[k][r]));
-        B[(i0 + i) * ldb + r] = v;
+    __syncthreads();
+    for (int idx = tid; idx < SB * MAXRHS; idx += 256) {
+        const int i = idx / MAXRHS, r = idx % MAXRHS;
+        acc[i][r] = tadd(tadd(part[0][i][r], part[1][i][r]), tadd(part[2][i][r], part[3][i][r]));
+    }
+    // the predecessor: y_b = acc - G_b y_pred
+    if (nprev > 0) {
+        if (tid == 0) wait_flag(order(nprev - 1));
+        __syncthreads();
+        stage_y(order(nprev - 1));
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < MAXRHS; ++r) {
+            T v = zero;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) v = tadd(v, tmul(g[u], ys[kq * 16 + u][r]));
+            part[kq][row][r] = v;
+        }
+        __syncthreads();
+        for (int idx = tid; idx < bn * nrhs; idx += 256) {
+            const int i = idx / nrhs, r = idx % nrhs;
+            B[(i0 + i) * ldb + r] =
+                tsub(acc[i][r], tadd(tadd(part[0][i][r], part[1][i][r]), tadd(part[2][i][r], part[3][i][r])));
+        }
+    } else {
+        __syncthreads();
+        for (int idx = tid; idx < bn * nrhs; idx += 256) {
+            const int i = idx / nrhs, r = idx % nrhs;
+            B[(i0 + i) * ldb + r] = acc[i][r];
+        }
     }
     __syncthreads();
     if (tid == 0) {
         __threadfence();
-        __hip_atomic_store(flags + b, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(flags + b, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -895,7 +756,7 @@ template <typename T> size_t tour_work_bytes(long n)
 }
 
 template <typename T>
-int getrf_tp(T *A, long n, long lda, int *ipiv, int *info, void *work, size_t wbytes, hipStream_t s)
+int getrf_tp(T *A, long n, long lda, int *ipiv, int *perm, int *info, void *work, size_t wbytes, hipStream_t s)
 {
     constexpr int RL = Tour<T>::RL, AR = Tour<T>::AR;
     if (wbytes < tour_work_bytes<T>(n))
@@ -907,28 +768,18 @@ int getrf_tp(T *A, long n, long lda, int *ipiv, int *info, void *work, size_t wb
     T *top = (T *)w, *Linv = top + TW * TW, *Uinv = Linv + TW * TW;
     w += 3 * (size_t)TW * TW * sizeof(T);
     int *arrive = (int *)w, *pairs = arrive + nn;
-    const size_t smem = (size_t)RL * TW * sizeof(T);
-    {
-        static bool attr_done[2] = {false, false};
-        static std::mutex mu;
-        std::lock_guard<std::mutex> lk(mu);
-        if (!attr_done[sizeof(T) == 8]) {
-            M25X_HIP(hipFuncSetAttribute((const void *)k_tslu<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-            attr_done[sizeof(T) == 8] = true;
-        }
-    }
     hipLaunchKernelGGL(k_zero_ints, dim3(blocks(nn, 256)), dim3(256), 0, s, arrive, (int)nn);
     if (info) hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, info, 1);
+    if (perm) hipLaunchKernelGGL(k_iota, dim3(blocks(n, 256)), dim3(256), 0, s, perm, n);
     if (int rc = check_launch("getrf")) return rc;
     for (long j0 = 0; j0 < n; j0 += TW) {
         const int jb = (int)std::min((long)TW, n - j0);
         const long m = n - j0;
         const int P = (int)tour_leaves(m, RL);
-        hipLaunchKernelGGL(k_tslu<T>, dim3(P), dim3(RL), (unsigned)smem, s, (const T *)A, lda, n, j0, jb, P, nodes,
-                           arrive, top, ipiv, pairs, info);
+        if (int rc = launch_tslu((const T *)A, lda, n, j0, jb, P, nodes, arrive, top, ipiv, pairs, info, s)) return rc;
         hipLaunchKernelGGL(k_trtri_top<T>, dim3(2), dim3(64), 0, s, (const T *)top, jb, Linv, Uinv);
-        hipLaunchKernelGGL(k_apply_moves<T>, dim3(blocks(n, 8)), dim3(256), 0, s, A, lda, 0L, n, (const int *)pairs,
-                           j0, jb, (const T *)top);
+        hipLaunchKernelGGL(k_apply_moves<T>, dim3(blocks(n, 8) + (perm ? 1 : 0)), dim3(256), 0, s, A, lda, 0L, n,
+                           (const int *)pairs, j0, jb, (const T *)top, perm);
         if (int rc = check_launch("getrf/panel")) return rc;
         if (m > jb) {   // L21 = A21 U11^-1
             T *A21 = A + j0 * lda + j0 + jb;
@@ -953,7 +804,14 @@ int getrf_tp(T *A, long n, long lda, int *ipiv, int *info, void *work, size_t wb
 template <typename T> size_t dinv_bytes(long n)
 {
     const long nb = (n + SB - 1) / SB;
-    return (size_t)nb * 2 * SB * SB * sizeof(T);
+    return (size_t)nb * NSLOT * SB * SB * sizeof(T);
+}
+
+template <typename T> void solve_ops(const T *LU, long lda, long n, T *dinv, hipStream_t s)
+{
+    const unsigned nb = (unsigned)((n + SB - 1) / SB);
+    hipLaunchKernelGGL(k_trtri_diag<T>, dim3(nb), dim3(SB), 0, s, LU, lda, n, dinv);
+    hipLaunchKernelGGL(k_solve_ops<T>, dim3(nb, 4), dim3(256), 0, s, LU, lda, n, dinv);
 }
 
 template <typename T> size_t getrs_work(long n, long ldb)
@@ -983,7 +841,7 @@ int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, 
         perm_in = perm;
     }
     if (!dinv) {
-        hipLaunchKernelGGL(k_trtri_diag<T>, dim3(nb), dim3(SB), 0, s, LU, lda, n, dwork);
+        solve_ops<T>(LU, lda, n, dwork, s);
         dinv = dwork;
     }
     if (int rc = check_launch("getrs")) return rc;
@@ -1013,8 +871,8 @@ size_t m25x_getrf_work_bytes(int dtype, long n)
                              : std::max(work_bytes<double2>(n), tour_work_bytes<double2>(n));
 }
 
-int m25x_getrf(int dtype, int pivot, void *A, long n, long lda, int *ipiv, int *info, void *work, size_t wbytes,
-               void *stream)
+int m25x_getrf(int dtype, int pivot, void *A, long n, long lda, int *ipiv, int *perm, int *info, void *work,
+               size_t wbytes, void *stream)
 {
     if (!A || !ipiv || !work || n < 0 || lda < n) return fail(M25X_EINVAL, "getrf: bad arguments");
     if (pivot != M25X_PIVOT_PARTIAL && pivot != M25X_PIVOT_TOURNAMENT) return fail(M25X_EINVAL, "getrf: pivot %d", pivot);
@@ -1022,13 +880,20 @@ int m25x_getrf(int dtype, int pivot, void *A, long n, long lda, int *ipiv, int *
     if (n > (1L << 30)) return fail(M25X_EINVAL, "getrf: n too large");
     hipStream_t s = (hipStream_t)stream;
     const bool tp = pivot == M25X_PIVOT_TOURNAMENT;
+    int rc;
     if (dtype == M25X_C128)
-        return tp ? getrf_tp<double2>((double2 *)A, n, lda, ipiv, info, work, wbytes, s)
-                  : getrf_t<double2>((double2 *)A, n, lda, ipiv, info, work, wbytes, s);
-    if (dtype == M25X_C64)
-        return tp ? getrf_tp<float2>((float2 *)A, n, lda, ipiv, info, work, wbytes, s)
-                  : getrf_t<float2>((float2 *)A, n, lda, ipiv, info, work, wbytes, s);
-    return fail(M25X_EINVAL, "getrf: dtype %d", dtype);
+        rc = tp ? getrf_tp<double2>((double2 *)A, n, lda, ipiv, perm, info, work, wbytes, s)
+                : getrf_t<double2>((double2 *)A, n, lda, ipiv, info, work, wbytes, s);
+    else if (dtype == M25X_C64)
+        rc = tp ? getrf_tp<float2>((float2 *)A, n, lda, ipiv, perm, info, work, wbytes, s)
+                : getrf_t<float2>((float2 *)A, n, lda, ipiv, info, work, wbytes, s);
+    else
+        return fail(M25X_EINVAL, "getrf: dtype %d", dtype);
+    if (rc == M25X_OK && perm && !tp) {
+        hipLaunchKernelGGL(k_perm, dim3(1), dim3(256), 0, s, (const int *)ipiv, n, perm);
+        rc = check_launch("getrf/perm");
+    }
+    return rc;
 }
 
 size_t m25x_getrs_work_bytes(int dtype, long n, long ldb)
@@ -1053,12 +918,11 @@ int m25x_trtri_blocks(int dtype, const void *LU, long n, long lda, void *dinv, v
 {
     if (!LU || !dinv || n < 0 || lda < n) return fail(M25X_EINVAL, "trtri_blocks: bad arguments");
     if (n == 0) return M25X_OK;
-    const unsigned nb = (unsigned)((n + SB - 1) / SB);
     hipStream_t s = (hipStream_t)stream;
     if (dtype == M25X_C128)
-        hipLaunchKernelGGL(k_trtri_diag<double2>, dim3(nb), dim3(SB), 0, s, (const double2 *)LU, lda, n, (double2 *)dinv);
+        solve_ops<double2>((const double2 *)LU, lda, n, (double2 *)dinv, s);
     else if (dtype == M25X_C64)
-        hipLaunchKernelGGL(k_trtri_diag<float2>, dim3(nb), dim3(SB), 0, s, (const float2 *)LU, lda, n, (float2 *)dinv);
+        solve_ops<float2>((const float2 *)LU, lda, n, (float2 *)dinv, s);
     else
         return fail(M25X_EINVAL, "trtri_blocks: dtype %d", dtype);
     return check_launch("trtri_blocks");

@@ -295,9 +295,8 @@ def getrf(Z, pivot='tournament'):
     wb = L.lib().m25x_getrf_work_bytes(dt, n)
     work = torch.empty(wb, dtype=torch.uint8, device=dv)
     pv = L.PIVOT_TOURNAMENT if pivot == 'tournament' else L.PIVOT_PARTIAL
-    L.call('m25x_getrf', dt, pv, ptr(Z), n, Z.stride(0), ptr(ipiv), ptr(info), ptr(work), wb, stream())
     perm = torch.empty(n, dtype=i32, device=dv)
-    L.call('m25x_ipiv_perm', ptr(ipiv), n, ptr(perm), stream())
+    L.call('m25x_getrf', dt, pv, ptr(Z), n, Z.stride(0), ptr(ipiv), ptr(perm), ptr(info), ptr(work), wb, stream())
     dinv = torch.empty(L.lib().m25x_dinv_bytes(dt, n), dtype=torch.uint8, device=dv)
     L.call('m25x_trtri_blocks', dt, ptr(Z), n, Z.stride(0), ptr(dinv), stream())
     return Factor(Z, ipiv, perm, info, dinv)

@@ -33,6 +33,7 @@ def test_getrf_c128_matches_lapack(gpu, n):
     lu = F.LU.cpu().numpy().T                              # our row-major buffer is A column-major
     piv = F.ipiv.cpu().numpy()
     assert np.array_equal(piv, piv_ref)
+    assert np.array_equal(F.perm.cpu().numpy(), olu.perm_of(piv))
     assert np.abs(lu - lu_ref).max() <= 1e-12 * np.abs(lu_ref).max()
     assert int(F.info.item()) == 0
     rng = np.random.default_rng(n)
@@ -114,6 +115,7 @@ def _check_tournament(dv, Z, F, RL, AR, tol_fac, tol_solve, full_ipiv):
     else:                                                  # the first panel's election, from the same inputs
         win = olu.tslu_winners(A, 0, min(64, n), RL, AR)
         assert np.array_equal(ipiv[:len(win)], olu.interchanges(win, 0, np.arange(n)))
+    assert np.array_equal(F.perm.cpu().numpy(), olu.perm_of(ipiv))
     lu = F.LU.cpu().numpy().T.astype(complex)
     Lf, Uf = np.tril(lu, -1) + np.eye(n), np.triu(lu)
     assert np.abs(A[olu.perm_of(ipiv)] - Lf @ Uf).max() <= tol_fac * np.abs(A).max()
