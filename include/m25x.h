@@ -156,21 +156,13 @@ int m25x_ipiv_perm(const int *ipiv, long n, int *perm, void *stream);
 /* Device workspace bytes m25x_getrs needs. */
 size_t m25x_getrs_work_bytes(int dtype, long n, long ldb);
 
-/* Device bytes of a factor's block solve operators (six 64 x 64 blocks per 64 rows). */
-size_t m25x_dinv_bytes(int dtype, long n);
-
-/* A factor's block solve operators, once per factor, for m25x_getrs: the inverses of its 64 x 64 diagonal blocks
- * (L_bb^-1, U_bb^-1) and, for each of the four triangular solves, the product of a block's diagonal inverse with
- * the off-diagonal block that couples it to the block solved before it. */
-int m25x_trtri_blocks(int dtype, const void *LU, long n, long lda, void *dinv, void *stream);
-
 /* Solves with the factor of m25x_getrf: trans = 1: Z X = B (A^T X = B, lu_solve(..., trans=1)), trans = 0:
  * A X = B. B: n x nrhs row-major (leading dimension ldb >= nrhs, 1 <= nrhs <= 8), the factor's dtype, overwritten
- * with X. perm: m25x_ipiv_perm's (or NULL: computed here); dinv: m25x_trtri_blocks' (or NULL: computed here).
- * Replaces the backend's solve (src/mom25d/gpu.py:76-80; src/mom25d/gpu_vk.py:650-678) and zgetrs / cgetrs
- * (src/mom25d/fast.py:356-358). */
-int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, const void *dinv,
-               void *B, long ldb, int nrhs, int trans, void *work, size_t work_bytes, void *stream);
+ * with X. perm: m25x_getrf's (or m25x_ipiv_perm's, or NULL: computed here). Diagonal blocks by substitution, as
+ * LAPACK's: the accuracy of a factor with small pivots is kept. Replaces the backend's solve (src/mom25d/gpu.py:76-80;
+ * src/mom25d/gpu_vk.py:650-678) and zgetrs / cgetrs (src/mom25d/fast.py:356-358). */
+int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B, long ldb,
+               int nrhs, int trans, void *work, size_t work_bytes, void *stream);
 
 /* ---- refinement products ------------------------------------------------------------------------------- */
 

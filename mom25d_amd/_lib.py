@@ -49,9 +49,7 @@ _SIGS = {
     'm25x_getrf': ([I, I, P, L, L, P, P, P, P, S, P], I),
     'm25x_ipiv_perm': ([P, L, P, P], I),
     'm25x_getrs_work_bytes': ([I, L, L], S),
-    'm25x_dinv_bytes': ([I, L], S),
-    'm25x_trtri_blocks': ([I, P, L, L, P, P], I),
-    'm25x_getrs': ([I, P, L, L, P, P, P, P, L, I, I, P, S, P], I),
+    'm25x_getrs': ([I, P, L, L, P, P, P, L, I, I, P, S, P], I),
     'm25x_zgemm_nk': ([P, L, L, P, I, P, P], I),
 }
 

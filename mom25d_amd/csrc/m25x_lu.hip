@@ -13,11 +13,11 @@
 //           (Re C = [Re A, Im A] [Re B; -Im B], Im C = [Re A, Im A] [Im B; Re B]), v_mfma_f64_16x16x4 for
 //           complex128 and v_mfma_f32_16x16x4 for complex64 (FP32 accumulation)
 // Tournament pivoting (CALU), panels of TW = 64 columns: the panel's pivots by a tree of GEPPs in one launch with no
-// grid barrier (k_tslu), its row moves on every column (k_apply_moves), then L21 = A21 U11^-1, U12 = L11^-1 A12 and
-// A22 -= L21 U12, all three on the matrix cores.
+// grid barrier (k_tslu), its row moves on every column (k_apply_moves), then L21 = A21 U11^-1 by substitution,
+// U12 = L11^-1 A12 by substitution and A22 -= L21 U12 on the matrix cores (complex64 with FP64 accumulation: one
+// rounding per update).
 // The solves (getrs) run one workgroup per 64-row block with ready flags between blocks (no launch per block), each
-// block's diagonal solve a product with its inverse and the step from its predecessor one precomputed 64 x 64
-// product (k_trtri_diag and k_solve_ops, once per factor).
+// block's diagonal solved by substitution in one wave.
 #include <mutex>
 #include "m25x_lu_dev.h"
 
@@ -262,12 +262,13 @@ __global__ __launch_bounds__(256) void k_trsm(T *A, long lda, long n, long j0, i
 // C's columns), its second A, so a lane's outputs run down C's columns (coalesced stores).
 constexpr int GM = 64, GN = 64, GK = 8;
 
-template <typename T, bool SUB>
+template <typename T, bool SUB, typename AC = T>
 __global__ __launch_bounds__(256) void k_gemm(T *C, long ldc, const T *A, long lda, const T *B, long ldb, long M,
                                               long N, long K)
 {
-    typedef typename CX<T>::R R;
-    typedef typename CX<T>::acc4 acc4;
+    typedef typename CX<AC>::R R;           // the matrix cores' type: products and sums in it, one rounding to T
+    typedef typename CX<T>::R TR;
+    typedef typename CX<AC>::acc4 acc4;
     __shared__ T As[GK][GM];         // As[p][i] = A[i][p]
     __shared__ T Bs[GK][GN + 1];     // Bs[p][j] = B[p][j]
     const long m0 = (long)blockIdx.x * GM, n0 = (long)blockIdx.y * GN;
@@ -281,18 +282,17 @@ __global__ __launch_bounds__(256) void k_gemm(T *C, long ldc, const T *A, long l
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) { cr[a][b][r] = 0; ci[a][b][r] = 0; }
-    const T zero = CX<T>::make(0, 0);
     for (long k0 = 0; k0 < K; k0 += GK) {
         // stage A (GM x GK) and B (GK x GN); zero outside
         for (int idx = tid; idx < GM * GK; idx += 256) {
             const int i = idx % GM, p = idx / GM;
             const long gi = m0 + i, gp = k0 + p;
-            As[p][i] = (gi < M && gp < K) ? A[gp * lda + gi] : zero;
+            As[p][i] = ldz(gi < M && gp < K, A + gp * lda + gi);
         }
         for (int idx = tid; idx < GK * GN; idx += 256) {
             const int p = idx % GK, j = idx / GK;
             const long gp = k0 + p, gj = n0 + j;
-            Bs[p][j] = (gp < K && gj < N) ? B[gj * ldb + gp] : zero;
+            Bs[p][j] = ldz(gp < K && gj < N, B + gj * ldb + gp);
         }
         __syncthreads();
 #pragma unroll
@@ -302,20 +302,20 @@ __global__ __launch_bounds__(256) void k_gemm(T *C, long ldc, const T *A, long l
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
                 const T x = As[p][wm + a * 16 + li];
-                av[a] = part ? x.y : x.x;
+                av[a] = (R)(part ? x.y : x.x);
             }
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
                 const T y = Bs[p][wn + b * 16 + li];
-                br[b] = part ? -y.y : y.x;
-                bi[b] = part ? y.x : y.y;
+                br[b] = (R)(part ? -y.y : y.x);
+                bi[b] = (R)(part ? y.x : y.y);
             }
 #pragma unroll
             for (int b = 0; b < 2; ++b)
 #pragma unroll
                 for (int a = 0; a < 2; ++a) {
-                    cr[b][a] = CX<T>::mfma(br[b], av[a], cr[b][a]);
-                    ci[b][a] = CX<T>::mfma(bi[b], av[a], ci[b][a]);
+                    cr[b][a] = CX<AC>::mfma(br[b], av[a], cr[b][a]);
+                    ci[b][a] = CX<AC>::mfma(bi[b], av[a], ci[b][a]);
                 }
         }
         __syncthreads();
@@ -327,14 +327,14 @@ __global__ __launch_bounds__(256) void k_gemm(T *C, long ldc, const T *A, long l
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const long gi = m0 + wm + a * 16 + li, gj = n0 + wn + b * 16 + CX<T>::out_row(kk, r);
+                const long gi = m0 + wm + a * 16 + li, gj = n0 + wn + b * 16 + CX<AC>::out_row(kk, r);
                 if (gi < M && gj < N) {
                     T *c = C + gj * ldc + gi;
                     if (SUB) {
                         const T v = *c;
-                        *c = CX<T>::make(v.x - cr[b][a][r], v.y - ci[b][a][r]);
+                        *c = CX<T>::make((TR)((R)v.x - cr[b][a][r]), (TR)((R)v.y - ci[b][a][r]));
                     } else {
-                        *c = CX<T>::make(cr[b][a][r], ci[b][a][r]);
+                        *c = CX<T>::make((TR)cr[b][a][r], (TR)ci[b][a][r]);
                     }
                 }
             }
@@ -368,86 +368,70 @@ __global__ void k_perm(const int *ipiv, long n, int *perm)
 }
 
 // ---- triangular solves, one workgroup a 64-row block, ready flags between blocks ---------------------------
-// B: n x nrhs row-major (ldb). The blocks of the solution are published in order; block b adds the
-// contributions of the blocks before it as their flags come up, then multiplies by the inverse of its diagonal
-// block (inverted once per factor by k_trtri_diag): no serial substitution on the critical path.
+// B: n x nrhs row-major (ldb). The blocks of the solution are published in order; block b takes the contributions
+// of the blocks before it as their flags come up, and solves its diagonal block by substitution (one wave, a row a
+// lane): an explicit inverse of a diagonal block loses the accuracy of a factor with small pivots, which the
+// refinement needs.
 constexpr int SB = 64;
 constexpr int MAXRHS = 8;
-// Per 64-row block b, NSLOT 64 x 64 row-major operators: 0 L_bb^-1, 1 U_bb^-1, 2 + kind the solve's step matrix
-// G_b = Mi_b Op_{b, pred(b)} of each kind (Mi_b the kind's diagonal inverse, pred(b) the block published before b).
-constexpr int NSLOT = 6;
 
-// The inverse of the unit lower (which 0) or upper (which 1) triangle of the bn x bn block at LU (column-major,
-// leading dimension lda), padded to 64 x 64 with the identity; out[i * SB + j] (row-major) or out[j * SB + i]
-// (column-major). 64 threads, one column of the inverse each.
+// L21 = A21 U11^-1 by substitution (x U11 = a for each row a of A21): a thread a row of [r0, r0 + m), U11 (the
+// upper part of top) in LDS; the 64 columns in two halves of 32 so a row's half stays in registers for either
+// element type.
 template <typename T>
-__device__ void trtri64(const T *LU, long lda, int bn, int which, T (*M)[SB + 1], T (*X)[SB + 1], T *out,
-                        bool colmajor)
+__global__ __launch_bounds__(256) void k_l21(T *A, long lda, long r0, long m, long j0, int jb, const T *top)
 {
-    const int j = threadIdx.x;
-    const T zero = CX<T>::make(0, 0), one = CX<T>::make(1, 0);
-    // M[i][k]: L (which 0, strictly lower part, unit diagonal) or U (which 1, upper part) of the block
-    for (int i = 0; i < SB; ++i) {
-        T v;
-        if (i < bn && j < bn) {
-            const T a = LU[(long)j * lda + i];
-            v = which == 0 ? (j < i ? a : (j == i ? one : zero)) : (j >= i ? a : zero);
-        } else {
-            v = (i == j) ? one : zero;
-        }
-        M[i][j] = v;
+    __shared__ T Us[TW][TW + 1];
+    __shared__ T rd[TW];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < TW * TW; e += 256) {
+        const int c = e / TW, i = e % TW;
+        Us[i][c] = top[e];
     }
     __syncthreads();
-    // sum_{k0 <= k < k1} M[i][k] X[k][j] with four independent partial sums (the loads of a group issued together)
-    auto dot = [&](int i, int k0, int k1) {
-        T s0 = zero, s1 = zero, s2 = zero, s3 = zero;
-        int k = k0;
-        for (; k + 4 <= k1; k += 4) {
-            s0 = tadd(s0, tmul(M[i][k], X[k][j]));
-            s1 = tadd(s1, tmul(M[i][k + 1], X[k + 1][j]));
-            s2 = tadd(s2, tmul(M[i][k + 2], X[k + 2][j]));
-            s3 = tadd(s3, tmul(M[i][k + 3], X[k + 3][j]));
+    if (tid < TW) rd[tid] = tid < jb ? trecip(Us[tid][tid]) : CX<T>::make(1, 0);
+    __syncthreads();
+    const long r = r0 + (long)blockIdx.x * 256 + tid;
+    if (r >= r0 + m) return;
+    T *row = A + j0 * lda + r;                  // element (r, j0 + c) at row[c * lda]
+    auto half = [&](auto hc) __attribute__((always_inline)) {
+        constexpr int h = decltype(hc)::value;
+        RowH<T> x;
+        auto ld = [&](auto jc) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
+            x.set(j, ldz(32 * h + j < jb, row + (long)(32 * h + j) * lda));
+        };
+        Unroll<0, TH>::run(ld);
+        if (h == 1) {                           // the first half's solution, final, from the previous pass
+            for (int k = 0; k < 32 && k < jb; ++k) {
+                const T xk = row[(long)k * lda];
+                auto up = [&](auto jc) __attribute__((always_inline)) {
+                    constexpr int j = decltype(jc)::value;
+                    x.set(j, tsub(x.get(j), tmul(xk, Us[k][32 + j])));
+                };
+                Unroll<0, TH>::run(up);
+            }
         }
-        for (; k < k1; ++k) s0 = tadd(s0, tmul(M[i][k], X[k][j]));
-        return tadd(tadd(s0, s1), tadd(s2, s3));
+        auto solve = [&](auto jc) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
+            if (32 * h + j < jb) {
+                T v = x.get(j);
+                auto sub = [&](auto kc) __attribute__((always_inline)) {
+                    constexpr int k = decltype(kc)::value;
+                    v = tsub(v, tmul(x.get(k), Us[32 * h + k][32 * h + j]));
+                };
+                Unroll<0, j>::run(sub);
+                x.set(j, tmul(v, rd[32 * h + j]));
+            }
+        };
+        Unroll<0, TH>::run(solve);
+        auto st = [&](auto jc) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
+            if (32 * h + j < jb) row[(long)(32 * h + j) * lda] = x.get(j);
+        };
+        Unroll<0, TH>::run(st);
     };
-    if (which == 0) {               // forward: X[i][j] = -sum_{j <= k < i} L[i][k] X[k][j]
-        for (int i = 0; i < SB; ++i)
-            X[i][j] = i < j ? zero : (i == j ? one : tsub(zero, dot(i, j, i)));
-    } else {                        // backward: X[i][j] = (delta_ij - sum_{i < k <= j} U[i][k] X[k][j]) / U[i][i]
-        for (int i = SB - 1; i >= 0; --i)
-            X[i][j] = i > j ? zero : tdiv(tsub(i == j ? one : zero, dot(i, i + 1, j + 1)), M[i][i]);
-    }
-    __syncthreads();
-    for (int i = 0; i < SB; ++i) {
-        if (colmajor) out[(long)j * SB + i] = X[i][j];
-        else out[(long)i * SB + j] = X[i][j];
-    }
-    __syncthreads();
-}
-
-// Per diagonal block b: Dinv[b][0] = L_bb^-1 (unit lower), Dinv[b][1] = U_bb^-1 (upper), 64 x 64 row-major;
-// a last block shorter than 64 is padded with the identity. One workgroup a block, one column a thread.
-template <typename T>
-__global__ __launch_bounds__(64) void k_trtri_diag(const T *LU, long lda, long n, T *Dinv)
-{
-    __shared__ T M[SB][SB + 1];
-    __shared__ T X[SB][SB + 1];
-    const long i0 = (long)blockIdx.x * SB;
-    const int bn = (int)std::min((long)SB, n - i0);
-    for (int which = 0; which < 2; ++which)
-        trtri64(LU + i0 * lda + i0, lda, bn, which, M, X, Dinv + ((size_t)blockIdx.x * NSLOT + which) * SB * SB, false);
-}
-
-// The inverses of the panel's L11 (block 0) and U11 (block 1), column-major 64 x 64, for L21 = A21 U11^-1 and
-// U12 = L11^-1 A12 on the matrix cores.
-template <typename T>
-__global__ __launch_bounds__(64) void k_trtri_top(const T *top, int jb, T *Linv, T *Uinv)
-{
-    __shared__ T M[SB][SB + 1];
-    __shared__ T X[SB][SB + 1];
-    const int which = blockIdx.x;
-    trtri64(top, TW, jb, which, M, X, which ? Uinv : Linv, true);
+    Unroll<0, 2>::run(half);
 }
 
 // The panel's row moves on columns [c0, c0 + nc): a column per 128 threads, every source read before any write;
@@ -483,86 +467,108 @@ __global__ __launch_bounds__(256) void k_apply_moves(T *A, long lda, long c0, lo
 
 // kind 0: U^T y = b (forward); 1: L^T z = y (backward); 2: L y = b (forward); 3: U x = y (backward).
 // Operator entry (row i, column k): kinds 0, 1 read LU[i * lda + k] (U[k][i], L[k][i]); kinds 2, 3 LU[k * lda + i].
-// Diagonal inverse Mi: kind 0 (U_bb^-1)^T, 1 (L_bb^-1)^T, 2 L_bb^-1, 3 U_bb^-1.
 __device__ inline bool kind_fwd(int kind) { return kind == 0 || kind == 2; }
 
-// The step matrices G_b = Mi_b Op_{b, pred(b)} (grid: blocks x 4 kinds), after k_trtri_diag.
-template <typename T>
-__global__ __launch_bounds__(256) void k_solve_ops(const T *LU, long lda, long n, T *Dinv)
+__device__ inline float2 readlane_c(float2 v, int l)
 {
-    __shared__ T Ms[SB][SB + 1];
-    __shared__ T Os[SB][SB + 1];
-    const int nb = (int)((n + SB - 1) / SB);
-    const int b = blockIdx.x, kind = blockIdx.y, tid = threadIdx.x;
-    const int p = kind_fwd(kind) ? b - 1 : b + 1;
-    T *G = Dinv + ((size_t)b * NSLOT + 2 + kind) * SB * SB;
-    const T zero = CX<T>::make(0, 0);
-    if (p < 0 || p >= nb) {
-        for (int idx = tid; idx < SB * SB; idx += 256) G[idx] = zero;
-        return;
-    }
-    const long i0 = (long)b * SB, k0 = (long)p * SB;
-    const int bn = (int)std::min((long)SB, n - i0), pn = (int)std::min((long)SB, n - k0);
-    const T *D = Dinv + ((size_t)b * NSLOT + ((kind == 0 || kind == 3) ? 1 : 0)) * SB * SB;
-    const bool tr = kind <= 1;
-    for (int idx = tid; idx < SB * SB; idx += 256) {
-        const int i = idx / SB, k = idx % SB;
-        Ms[i][k] = tr ? D[k * SB + i] : D[i * SB + k];
-        Os[i][k] = (i < bn && k < pn) ? (tr ? LU[(i0 + i) * lda + k0 + k] : LU[(k0 + k) * lda + i0 + i]) : zero;
-    }
-    __syncthreads();
-    for (int idx = tid; idx < SB * SB; idx += 256) {
-        const int i = idx / SB, j = idx % SB;
-        T g0 = zero, g1 = zero, g2 = zero, g3 = zero;
-        for (int k = 0; k < SB; k += 4) {
-            g0 = tadd(g0, tmul(Ms[i][k], Os[k][j]));
-            g1 = tadd(g1, tmul(Ms[i][k + 1], Os[k + 1][j]));
-            g2 = tadd(g2, tmul(Ms[i][k + 2], Os[k + 2][j]));
-            g3 = tadd(g3, tmul(Ms[i][k + 3], Os[k + 3][j]));
-        }
-        G[idx] = tadd(tadd(g0, g1), tadd(g2, g3));
-    }
+    return make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), l)),
+                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), l)));
+}
+__device__ inline double readlane_d(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline double2 readlane_c(double2 v, int l) { return make_double2(readlane_d(v.x, l), readlane_d(v.y, l)); }
+
+// t - m y with four fused multiply-adds
+__device__ inline float2 cfms(float2 t, float2 m, float2 y)
+{
+    return make_float2(fmaf(-m.x, y.x, fmaf(m.y, y.y, t.x)), fmaf(-m.y, y.x, fmaf(-m.x, y.y, t.y)));
+}
+__device__ inline double2 cfms(double2 t, double2 m, double2 y)
+{
+    return make_double2(fma(-m.x, y.x, fma(m.y, y.y, t.x)), fma(-m.y, y.x, fma(-m.x, y.y, t.y)));
 }
 
-// Workgroup b (in publication order) owns a 64-row block: y_b = Mi_b (b_b - sum_{a before pred} Op_ba y_a)
-// - G_b y_pred. The sum is taken block by block as the flags come up (thread (row, slice) keeps partial sums over
-// its 16-column slice), and Mi_b applied, before the predecessor publishes; after its flag the critical path is one
-// 64 x 64 product with G_b (held in registers), a reduction and the publish.
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsv(const T *LU, long lda, long n, const T *Dinv, T *B, long ldb, int nrhs,
-                                              int kind, int *flags, int *abort)
+// Unit-diagonal substitution of one right-hand side held a row a lane: step k broadcasts row k's value and the rows
+// after it (FWD) or before it subtract their multiple; the operator's row comes from LDS eight steps ahead.
+template <bool FWD, typename T>
+__device__ __attribute__((always_inline)) T subst(T t, const T (*Ms)[SB + 1], int bn, int lane)
 {
+    for (int s0 = 0; s0 < bn; s0 += 8) {
+        T mk[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int step = s0 + j, k = FWD ? step : bn - 1 - step;
+            mk[j] = ldz(step < bn, &Ms[lane][k < 0 ? 0 : k]);
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int step = s0 + j;
+            if (step < bn) {
+                const int k = FWD ? step : bn - 1 - step;
+                const T y = readlane_c(t, k);
+                t = csel(FWD ? lane > k : lane < k, cfms(t, mk[j], y), t);
+            }
+        }
+    }
+    return t;
+}
+
+// Workgroup b (in publication order) owns a 64-row block: t = b_b - sum over the blocks solved before it of
+// Op_ba y_a, taken block by block as the flags come up (thread (row, slice) keeps partial sums over its 16-column
+// slice; the immediate predecessor's entries are in registers before its flag), then Op_bb y_b = t by
+// substitution in one wave, then the publish.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv(const T *LU, long lda, long n, T *B, long ldb, int nrhs, int kind,
+                                              int *flags, int *abort)
+{
+    __shared__ T Ms[SB][SB + 1];             // the operator's diagonal block, Ms[i][k]
+    __shared__ T rdiag[SB];                  // 1 / its diagonal (1 for the unit triangles)
     __shared__ T acc[SB][MAXRHS];
     __shared__ T ys[SB][MAXRHS];
     __shared__ T part[4][SB][MAXRHS];
     __shared__ int s_avail;
     const int nb = (int)((n + SB - 1) / SB);
     const bool fwd = kind_fwd(kind);
+    const bool unit = kind == 1 || kind == 2;
     const int b = fwd ? blockIdx.x : nb - 1 - blockIdx.x;     // publication order is blockIdx order
     const long i0 = (long)b * SB;
     const int bn = (int)std::min((long)SB, n - i0);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
     const bool rowmajor = kind <= 1;
     const T zero = CX<T>::make(0, 0);
     const int row = tid & 63, kq = tid >> 6;                  // 4 slices of 16 columns
-    T g[16];
-    const T *D = Dinv + ((size_t)b * NSLOT + ((kind == 0 || kind == 3) ? 1 : 0)) * SB * SB;
-    {
-        const T *Gb = Dinv + ((size_t)b * NSLOT + 2 + kind) * SB * SB;
-#pragma unroll
-        for (int u = 0; u < 16; ++u) g[u] = Gb[row * SB + kq * 16 + u];
+    for (int idx = tid; idx < SB * SB; idx += 256) {
+        const int i = rowmajor ? idx / SB : idx % SB, k = rowmajor ? idx % SB : idx / SB;
+        Ms[i][k] = ldz(i < bn && k < bn, rowmajor ? LU + (i0 + i) * lda + i0 + k : LU + (i0 + k) * lda + i0 + i);
     }
     for (int idx = tid; idx < SB * MAXRHS; idx += 256) {
         const int i = idx / MAXRHS, r = idx % MAXRHS;
-        acc[i][r] = (i < bn && r < nrhs) ? B[(i0 + i) * ldb + r] : zero;
+        acc[i][r] = ldz(i < bn && r < nrhs, B + (i0 + i) * ldb + r);
+    }
+    __syncthreads();
+    if (tid < SB) rdiag[tid] = (unit || tid >= bn) ? CX<T>::make(1, 0) : trecip(Ms[tid][tid]);
+    if (!unit) {                             // rows scaled by the reciprocal diagonal: a unit-diagonal substitution
+        __syncthreads();
+        for (int idx = tid; idx < SB * SB; idx += 256) {
+            const int i = idx / SB, k = idx % SB;
+            Ms[i][k] = tmul(Ms[i][k], rdiag[i]);
+        }
     }
     auto order = [&](int q) { return fwd ? q : nb - 1 - q; };
+    auto entry = [&](int a, int u) -> T {
+        const long gi = i0 + row, gk = (long)a * SB + kq * 16 + u;
+        return ldz(row < bn && gk < n, rowmajor ? LU + gi * lda + gk : LU + gk * lda + gi);
+    };
     auto stage_y = [&](int a) {              // ys = y_a, zero padded
         const long k0 = (long)a * SB;
         const int kn = (int)std::min((long)SB, n - k0);
         for (int idx = tid; idx < SB * MAXRHS; idx += 256) {
             const int i = idx / MAXRHS, r = idx % MAXRHS;
-            ys[i][r] = (i < kn && r < nrhs) ? B[(k0 + i) * ldb + r] : zero;
+            ys[i][r] = ldz(i < kn && r < nrhs, B + (k0 + i) * ldb + r);
         }
     };
     auto wait_flag = [&](int a) {
@@ -572,30 +578,28 @@ __global__ __launch_bounds__(256) void k_trsv(const T *LU, long lda, long n, con
             if (++spins > (1L << 26)) { __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
         }
     };
+    const int nprev = (int)blockIdx.x;
+    T ml[16];                                 // the immediate predecessor's entries
+#pragma unroll
+    for (int u = 0; u < 16; ++u) ml[u] = nprev > 0 ? entry(order(nprev - 1), u) : CX<T>::make(0, 0);
     T s[MAXRHS];
 #pragma unroll
     for (int r = 0; r < MAXRHS; ++r) s[r] = zero;
-    const int nprev = (int)blockIdx.x;
-    const int nbulk = nprev > 0 ? nprev - 1 : 0;
     int done = 0;
-    while (done < nbulk) {
+    while (done < nprev) {
         if (tid == 0) {
             wait_flag(order(done));
             int q = done + 1;
-            while (q < nbulk && __hip_atomic_load(flags + order(q), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0) ++q;
+            while (q < nprev && __hip_atomic_load(flags + order(q), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0) ++q;
             s_avail = q;
         }
         __syncthreads();
         const int avail = s_avail;
         for (int q = done; q < avail; ++q) {
             const int a = order(q);
-            const long k0 = (long)a * SB;
             T m[16];
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const long gi = i0 + row, gk = k0 + kq * 16 + u;
-                m[u] = (row < bn && gk < n) ? (rowmajor ? LU[gi * lda + gk] : LU[gk * lda + gi]) : zero;
-            }
+            for (int u = 0; u < 16; ++u) m[u] = csel(q == nprev - 1, ml[u], entry(a, u));
             stage_y(a);
             __syncthreads();
 #pragma unroll
@@ -606,60 +610,16 @@ __global__ __launch_bounds__(256) void k_trsv(const T *LU, long lda, long n, con
         }
         done = avail;
     }
-    // acc <- Mi (acc - sum)
 #pragma unroll
     for (int r = 0; r < MAXRHS; ++r) part[kq][row][r] = s[r];
     __syncthreads();
-    for (int idx = tid; idx < SB * MAXRHS; idx += 256) {
-        const int i = idx / MAXRHS, r = idx % MAXRHS;
-        acc[i][r] = tsub(acc[i][r], tadd(tadd(part[0][i][r], part[1][i][r]), tadd(part[2][i][r], part[3][i][r])));
-    }
-    __syncthreads();
-    {
-        T mi[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int k = kq * 16 + u;
-            mi[u] = rowmajor ? D[k * SB + row] : D[row * SB + k];
-        }
-#pragma unroll
-        for (int r = 0; r < MAXRHS; ++r) {
-            T v = zero;
-#pragma unroll
-            for (int u = 0; u < 16; ++u) v = tadd(v, tmul(mi[u], acc[kq * 16 + u][r]));
-            part[kq][row][r] = v;
-        }
-    }
-    __syncthreads();
-    for (int idx = tid; idx < SB * MAXRHS; idx += 256) {
-        const int i = idx / MAXRHS, r = idx % MAXRHS;
-        acc[i][r] = tadd(tadd(part[0][i][r], part[1][i][r]), tadd(part[2][i][r], part[3][i][r]));
-    }
-    // the predecessor: y_b = acc - G_b y_pred
-    if (nprev > 0) {
-        if (tid == 0) wait_flag(order(nprev - 1));
-        __syncthreads();
-        stage_y(order(nprev - 1));
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < MAXRHS; ++r) {
-            T v = zero;
-#pragma unroll
-            for (int u = 0; u < 16; ++u) v = tadd(v, tmul(g[u], ys[kq * 16 + u][r]));
-            part[kq][row][r] = v;
-        }
-        __syncthreads();
-        for (int idx = tid; idx < bn * nrhs; idx += 256) {
-            const int i = idx / nrhs, r = idx % nrhs;
-            B[(i0 + i) * ldb + r] =
-                tsub(acc[i][r], tadd(tadd(part[0][i][r], part[1][i][r]), tadd(part[2][i][r], part[3][i][r])));
-        }
-    } else {
-        __syncthreads();
-        for (int idx = tid; idx < bn * nrhs; idx += 256) {
-            const int i = idx / nrhs, r = idx % nrhs;
-            B[(i0 + i) * ldb + r] = acc[i][r];
-        }
+    // t = acc - sum, then the substitution: wave w takes right-hand sides w and w + 4, lane i row i; with the rows
+    // scaled to a unit diagonal a step is one broadcast and one multiply-subtract
+    for (int r = tid >> 6; r < nrhs; r += 4) {
+        T t = tmul(tsub(acc[lane][r], tadd(tadd(part[0][lane][r], part[1][lane][r]),
+                                            tadd(part[2][lane][r], part[3][lane][r]))), rdiag[lane]);
+        t = fwd ? subst<true>(t, Ms, bn, lane) : subst<false>(t, Ms, bn, lane);
+        if (lane < bn) B[(i0 + lane) * ldb + r] = t;
     }
     __syncthreads();
     if (tid == 0) {
@@ -752,7 +712,7 @@ int getrf_t(T *A, long n, long lda, int *ipiv, int *info, void *work, size_t wby
 template <typename T> size_t tour_work_bytes(long n)
 {
     const long nn = tour_nodes(n, Tour<T>::RL, Tour<T>::AR);
-    return nn * sizeof(TNode<T>) + 3 * (size_t)TW * TW * sizeof(T) + sizeof(int) * ((size_t)nn + 4 * TW + 64);
+    return nn * sizeof(TNode<T>) + (size_t)TW * TW * sizeof(T) + sizeof(int) * ((size_t)nn + 4 * TW + 64);
 }
 
 template <typename T>
@@ -765,8 +725,8 @@ int getrf_tp(T *A, long n, long lda, int *ipiv, int *perm, int *info, void *work
     char *w = (char *)work;
     TNode<T> *nodes = (TNode<T> *)w;
     w += nn * sizeof(TNode<T>);
-    T *top = (T *)w, *Linv = top + TW * TW, *Uinv = Linv + TW * TW;
-    w += 3 * (size_t)TW * TW * sizeof(T);
+    T *top = (T *)w;
+    w += (size_t)TW * TW * sizeof(T);
     int *arrive = (int *)w, *pairs = arrive + nn;
     hipLaunchKernelGGL(k_zero_ints, dim3(blocks(nn, 256)), dim3(256), 0, s, arrive, (int)nn);
     if (info) hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, info, 1);
@@ -777,22 +737,18 @@ int getrf_tp(T *A, long n, long lda, int *ipiv, int *perm, int *info, void *work
         const long m = n - j0;
         const int P = (int)tour_leaves(m, RL);
         if (int rc = launch_tslu((const T *)A, lda, n, j0, jb, P, nodes, arrive, top, ipiv, pairs, info, s)) return rc;
-        hipLaunchKernelGGL(k_trtri_top<T>, dim3(2), dim3(64), 0, s, (const T *)top, jb, Linv, Uinv);
         hipLaunchKernelGGL(k_apply_moves<T>, dim3(blocks(n, 8) + (perm ? 1 : 0)), dim3(256), 0, s, A, lda, 0L, n,
                            (const int *)pairs, j0, jb, (const T *)top, perm);
         if (int rc = check_launch("getrf/panel")) return rc;
-        if (m > jb) {   // L21 = A21 U11^-1
-            T *A21 = A + j0 * lda + j0 + jb;
-            hipLaunchKernelGGL((k_gemm<T, false>), dim3(blocks(m - jb, GM), 1), dim3(256), 0, s, A21, lda,
-                               (const T *)A21, lda, (const T *)Uinv, (long)TW, m - jb, (long)jb, (long)jb);
-        }
+        if (m > jb)     // L21 = A21 U11^-1
+            hipLaunchKernelGGL(k_l21<T>, dim3(blocks(m - jb, 256)), dim3(256), 0, s, A, lda, j0 + jb, m - jb, j0, jb,
+                               (const T *)top);
         const long c0 = j0 + jb;
         if (c0 < n) {   // U12 = L11^-1 A12, A22 -= L21 U12
             T *A12 = A + c0 * lda + j0;
-            hipLaunchKernelGGL((k_gemm<T, false>), dim3(1, blocks(n - c0, GN)), dim3(256), 0, s, A12, lda,
-                               (const T *)Linv, (long)TW, (const T *)A12, lda, (long)jb, n - c0, (long)jb);
+            hipLaunchKernelGGL(k_trsm<T>, dim3(blocks(n - c0, TC)), dim3(256), 0, s, A, lda, n, j0, jb, c0);
             const long M = n - c0;
-            hipLaunchKernelGGL((k_gemm<T, true>), dim3(blocks(M, GM), blocks(M, GN)), dim3(256), 0, s,
+            hipLaunchKernelGGL((k_gemm<T, true, double2>), dim3(blocks(M, GM), blocks(M, GN)), dim3(256), 0, s,
                                A + c0 * lda + c0, lda, (const T *)(A + j0 * lda + c0), lda, (const T *)A12, lda, M, M,
                                (long)jb);
         }
@@ -801,28 +757,15 @@ int getrf_tp(T *A, long n, long lda, int *ipiv, int *perm, int *info, void *work
     return M25X_OK;
 }
 
-template <typename T> size_t dinv_bytes(long n)
-{
-    const long nb = (n + SB - 1) / SB;
-    return (size_t)nb * NSLOT * SB * SB * sizeof(T);
-}
-
-template <typename T> void solve_ops(const T *LU, long lda, long n, T *dinv, hipStream_t s)
-{
-    const unsigned nb = (unsigned)((n + SB - 1) / SB);
-    hipLaunchKernelGGL(k_trtri_diag<T>, dim3(nb), dim3(SB), 0, s, LU, lda, n, dinv);
-    hipLaunchKernelGGL(k_solve_ops<T>, dim3(nb, 4), dim3(256), 0, s, LU, lda, n, dinv);
-}
-
 template <typename T> size_t getrs_work(long n, long ldb)
 {
     const long nb = (n + SB - 1) / SB;
-    return (size_t)n * ldb * sizeof(T) + dinv_bytes<T>(n) + sizeof(int) * ((size_t)n + 2 * nb + 64);
+    return (size_t)n * ldb * sizeof(T) + sizeof(int) * ((size_t)n + 2 * nb + 64);
 }
 
 template <typename T>
-int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, const T *dinv, T *B, long ldb,
-            int nrhs, int trans, void *work, size_t wbytes, hipStream_t s)
+int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, T *B, long ldb, int nrhs, int trans,
+            void *work, size_t wbytes, hipStream_t s)
 {
     if (nrhs < 1 || nrhs > MAXRHS) return fail(M25X_EINVAL, "getrs: 1 to %d right-hand sides, not %d", MAXRHS, nrhs);
     if (ldb < nrhs) return fail(M25X_EINVAL, "getrs: ldb < nrhs");
@@ -831,8 +774,7 @@ int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, 
     const int nb = (int)((n + SB - 1) / SB);
     const size_t tmp_bytes = (size_t)n * ldb * sizeof(T);
     T *tmp = (T *)work;
-    T *dwork = (T *)((char *)work + tmp_bytes);
-    int *perm = (int *)((char *)dwork + dinv_bytes<T>(n));
+    int *perm = (int *)((char *)work + tmp_bytes);
     int *flags = perm + n;
     int *abort = flags + 2 * nb;
     hipLaunchKernelGGL(k_zero_ints, dim3(blocks(2 * nb + 1, 256)), dim3(256), 0, s, flags, 2 * nb + 1);
@@ -840,21 +782,17 @@ int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, 
         hipLaunchKernelGGL(k_perm, dim3(1), dim3(256), 0, s, ipiv, n, perm);
         perm_in = perm;
     }
-    if (!dinv) {
-        solve_ops<T>(LU, lda, n, dwork, s);
-        dinv = dwork;
-    }
     if (int rc = check_launch("getrs")) return rc;
     if (trans) {
-        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, dinv, B, ldb, nrhs, 0, flags, abort);
-        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, dinv, B, ldb, nrhs, 1, flags + nb, abort);
+        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, B, ldb, nrhs, 0, flags, abort);
+        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, B, ldb, nrhs, 1, flags + nb, abort);
         hipLaunchKernelGGL(k_permute<T>, dim3(blocks(n * nrhs, 256)), dim3(256), 0, s, (const T *)B, tmp, n, nrhs, ldb,
                            perm_in, 1);
     } else {
         hipLaunchKernelGGL(k_permute<T>, dim3(blocks(n * nrhs, 256)), dim3(256), 0, s, (const T *)B, tmp, n, nrhs, ldb,
                            perm_in, 0);
-        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, dinv, tmp, ldb, nrhs, 2, flags, abort);
-        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, dinv, tmp, ldb, nrhs, 3, flags + nb, abort);
+        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, tmp, ldb, nrhs, 2, flags, abort);
+        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, tmp, ldb, nrhs, 3, flags + nb, abort);
     }
     if (int rc = check_launch("getrs")) return rc;
     M25X_HIP(hipMemcpyAsync(B, tmp, tmp_bytes, hipMemcpyDeviceToDevice, s));
@@ -909,37 +847,18 @@ int m25x_ipiv_perm(const int *ipiv, long n, int *perm, void *stream)
     return check_launch("ipiv_perm");
 }
 
-size_t m25x_dinv_bytes(int dtype, long n)
-{
-    return dtype == M25X_C64 ? dinv_bytes<float2>(n) : dinv_bytes<double2>(n);
-}
-
-int m25x_trtri_blocks(int dtype, const void *LU, long n, long lda, void *dinv, void *stream)
-{
-    if (!LU || !dinv || n < 0 || lda < n) return fail(M25X_EINVAL, "trtri_blocks: bad arguments");
-    if (n == 0) return M25X_OK;
-    hipStream_t s = (hipStream_t)stream;
-    if (dtype == M25X_C128)
-        solve_ops<double2>((const double2 *)LU, lda, n, (double2 *)dinv, s);
-    else if (dtype == M25X_C64)
-        solve_ops<float2>((const float2 *)LU, lda, n, (float2 *)dinv, s);
-    else
-        return fail(M25X_EINVAL, "trtri_blocks: dtype %d", dtype);
-    return check_launch("trtri_blocks");
-}
-
-int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, const void *dinv,
-               void *B, long ldb, int nrhs, int trans, void *work, size_t work_bytes, void *stream)
+int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B, long ldb,
+               int nrhs, int trans, void *work, size_t work_bytes, void *stream)
 {
     if (!LU || !ipiv || !B || n < 0 || lda < n) return fail(M25X_EINVAL, "getrs: bad arguments");
     if (n == 0) return M25X_OK;
     hipStream_t s = (hipStream_t)stream;
     if (dtype == M25X_C128)
-        return getrs_t<double2>((const double2 *)LU, n, lda, ipiv, perm, (const double2 *)dinv, (double2 *)B, ldb, nrhs,
-                                trans, work, work_bytes, s);
+        return getrs_t<double2>((const double2 *)LU, n, lda, ipiv, perm, (double2 *)B, ldb, nrhs, trans, work,
+                                work_bytes, s);
     if (dtype == M25X_C64)
-        return getrs_t<float2>((const float2 *)LU, n, lda, ipiv, perm, (const float2 *)dinv, (float2 *)B, ldb, nrhs,
-                               trans, work, work_bytes, s);
+        return getrs_t<float2>((const float2 *)LU, n, lda, ipiv, perm, (float2 *)B, ldb, nrhs, trans, work, work_bytes,
+                               s);
     return fail(M25X_EINVAL, "getrs: dtype %d", dtype);
 }
 

@@ -57,6 +57,18 @@ template <typename T> __device__ inline T tdiv(T a, T b)
     R r = b.x / b.y, den = b.x * r + b.y;
     return CX<T>::make((a.x * r + a.y) / den, (a.y * r - a.x) / den);
 }
+// *p where c holds, else zero: a guarded load (a select between the loaded value and a zero constant is lowered as a
+// select of addresses, with the zero kept in scratch memory and the load through a generic pointer)
+template <typename T> __device__ __attribute__((always_inline)) inline T ldz(bool c, const T *p)
+{
+    T v = CX<T>::make(0, 0);
+    if (c) v = *p;
+    return v;
+}
+
+// a ? x : y component by component (a select of the vector-typed complex itself is lowered through scratch memory
+// for complex128)
+template <typename T> __device__ inline T csel(bool c, T a, T b) { return CX<T>::make(c ? a.x : b.x, c ? a.y : b.y); }
 template <typename T> __device__ inline double cabs1(T a) { return fabs((double)a.x) + fabs((double)a.y); }
 
 // ---- tournament pivoting (communication-avoiding LU panels) -----------------------------------------------
@@ -85,6 +97,29 @@ inline long tour_nodes(long n, int RL, int AR)
     while (P > 1) { tot += P; P = (P + AR - 1) / AR; }
     return tot + 1;
 }
+
+// Calls f(integral_constant<int, K>) for K = B .. E - 1: the steps of an elimination with every register index a
+// compile-time constant (a row held in registers is never indexed at run time, so it stays in registers).
+template <int B, int E> struct Unroll {
+    template <class F> __device__ __attribute__((always_inline)) static void run(F &f)
+    {
+        f(std::integral_constant<int, B>{});
+        Unroll<B + 1, E>::run(f);
+    }
+};
+template <int E> struct Unroll<E, E> {
+    template <class F> __device__ __attribute__((always_inline)) static void run(F &) {}
+};
+
+constexpr int TH = TW / 2;
+
+// A row's half in registers as separate real and imaginary arrays (an array of the vector-typed complex is kept
+// in scratch memory for complex128).
+template <typename T> struct RowH {
+    typename CX<T>::R re[TH], im[TH];
+    __device__ T get(int j) const { return CX<T>::make(re[j], im[j]); }
+    __device__ void set(int j, T v) { re[j] = v.x; im[j] = v.y; }
+};
 
 // One tournament launch (k_tslu) for panel [j0, j0 + jb): P leaves; defined in m25x_tslu.hip, one build per type.
 int launch_tslu(const double2 *A, long lda, long n, long j0, int jb, int P, TNode<double2> *nodes, int *arrive,

@@ -6,79 +6,69 @@ namespace m25x {
 namespace lu {
 namespace {
 
-// Wave argmax of (v, key): the larger v, then the smaller key; every lane ends with the result. Within each row
-// of 16 lanes by DPP exchanges (quad xor 1, quad xor 2, half-row mirror, row mirror), then the four rows' results
-// combined in a fixed order.
-__device__ inline void better_key(double &v, int &key, double v2, int k2)
+// The pivot search orders candidates by (magnitude, then the smaller key): magnitudes |Re| + |Im| are non-negative,
+// so their IEEE bit patterns order as unsigned integers (a dead row has bits 0 and the largest key, so any live row,
+// even one with a zero entry, outranks it). Wave reductions in two passes - the largest bits, then the smallest
+// key among the lanes holding them - each within rows of 16 lanes by DPP exchanges (quad xor 1, quad xor 2, half-row
+// mirror, row mirror), then over the four rows by readlane.
+typedef unsigned long long u64;
+
+template <int CTRL> __device__ inline u64 dpp_u64(u64 v)
 {
-    if (v2 > v || (v2 == v && k2 < key)) { v = v2; key = k2; }
+    const unsigned lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)v, CTRL, 0xF, 0xF, false);
+    const unsigned hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), CTRL, 0xF, 0xF, false);
+    return ((u64)hi << 32) | lo;
+}
+__device__ inline u64 readlane_u64(u64 v, int l)
+{
+    const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)v, l), hi = __builtin_amdgcn_readlane((int)(unsigned)(v >> 32), l);
+    return ((u64)hi << 32) | lo;
 }
 
-template <int CTRL> __device__ inline void dpp_better(double &v, int &key)
+__device__ inline void wave_best(u64 &vb, int &key)
 {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-    const int k2 = __builtin_amdgcn_update_dpp(0, key, CTRL, 0xF, 0xF, false);
-    better_key(v, key, __longlong_as_double(((long long)hi << 32) | (unsigned)lo), k2);
+    u64 m = vb;
+    m = max(m, dpp_u64<0xB1>(m));      // quad_perm [1, 0, 3, 2]
+    m = max(m, dpp_u64<0x4E>(m));      // quad_perm [2, 3, 0, 1]
+    m = max(m, dpp_u64<0x141>(m));     // row_half_mirror
+    m = max(m, dpp_u64<0x140>(m));     // row_mirror
+    m = max(max(readlane_u64(m, 0), readlane_u64(m, 16)), max(readlane_u64(m, 32), readlane_u64(m, 48)));
+    int k = vb == m ? key : 0x7fffffff;
+    k = min(k, __builtin_amdgcn_update_dpp(0, k, 0xB1, 0xF, 0xF, false));
+    k = min(k, __builtin_amdgcn_update_dpp(0, k, 0x4E, 0xF, 0xF, false));
+    k = min(k, __builtin_amdgcn_update_dpp(0, k, 0x141, 0xF, 0xF, false));
+    k = min(k, __builtin_amdgcn_update_dpp(0, k, 0x140, 0xF, 0xF, false));
+    k = min(min(__builtin_amdgcn_readlane(k, 0), __builtin_amdgcn_readlane(k, 16)),
+            min(__builtin_amdgcn_readlane(k, 32), __builtin_amdgcn_readlane(k, 48)));
+    vb = m;
+    key = k;
 }
 
-__device__ inline void wave_argmax(double &v, int &key)
-{
-    dpp_better<0xB1>(v, key);       // quad_perm [1, 0, 3, 2]
-    dpp_better<0x4E>(v, key);       // quad_perm [2, 3, 0, 1]
-    dpp_better<0x141>(v, key);      // row_half_mirror
-    dpp_better<0x140>(v, key);      // row_mirror
-    const long long b = __double_as_longlong(v);
-    double bv = -2.0;
-    int bk = 0x7fffffff;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int lo = __builtin_amdgcn_readlane((int)b, 16 * r), hi = __builtin_amdgcn_readlane((int)(b >> 32), 16 * r);
-        better_key(bv, bk, __longlong_as_double(((long long)hi << 32) | (unsigned)lo),
-                   __builtin_amdgcn_readlane(key, 16 * r));
-    }
-    v = bv;
-    key = bk;
-}
-
-// Calls f(integral_constant<int, K>) for K = B .. E - 1: the steps of an elimination with every register index a
-// compile-time constant (a row held in registers is never indexed at run time, so it stays in registers).
-template <int B, int E> struct Unroll {
-    template <class F> __device__ __attribute__((always_inline)) static void run(F &f)
-    {
-        f(std::integral_constant<int, B>{});
-        Unroll<B + 1, E>::run(f);
-    }
-};
-template <int E> struct Unroll<E, E> {
-    template <class F> __device__ __attribute__((always_inline)) static void run(F &) {}
-};
+__device__ inline u64 mag_bits(float2 a) { return (u64)__float_as_uint(fabsf(a.x) + fabsf(a.y)); }
+__device__ inline u64 mag_bits(double2 a) { return (u64)__double_as_longlong(fabs(a.x) + fabs(a.y)); }
 
 // GEPP of rows held in registers, two threads a row: in wave w, lanes L and L + 32 hold row 32 w + L % 32, the
 // thread of half h = L / 32 columns 2 j + h in x[j] (j < TW / 2), so both halves have equal work at every step.
 // nr rows over columns [0, nc). Rows never move: a row's position in the elimination order is tracked instead
-// (pos), and ties in the pivot search go to the earliest position (LAPACK's rule). Step k: the pivot row publishes
-// itself in prow (LDS), the live rows take their multiplier and update. order[k] = the k-th pivot row; a pivot
-// row's threads keep its step in step_of. On return a pivot row holds its row of L \ U, the other rows their
-// multipliers and Schur complement. Two barriers a step; the per-wave maxima double-buffered.
-constexpr int TH = TW / 2;
+// (pos), and ties in the pivot search go to the earliest position (LAPACK's rule). Step k: every wave's best row
+// publishes itself in its slot (LDS) with its (magnitude, key); after one barrier every thread picks the best of
+// the slots, takes its multiplier and updates. order[k] = the k-th pivot row; a pivot row's threads keep its step in
+// step_of. On return a pivot row holds its row of L \ U, the other rows their multipliers and Schur complement.
+// One barrier a step: the slots are double-buffered.
 template <typename T> __device__ inline T shfl_xor32(T v)
 {
     return CX<T>::make(__shfl_xor(v.x, 32), __shfl_xor(v.y, 32));
 }
 
-// A row's half in registers as separate real and imaginary arrays (an array of the vector-typed complex is kept
-// in scratch memory for complex128).
-template <typename T> struct RowH {
-    typename CX<T>::R re[TH], im[TH];
-    __device__ T get(int j) const { return CX<T>::make(re[j], im[j]); }
-    __device__ void set(int j, T v) { re[j] = v.x; im[j] = v.y; }
+template <typename T, int NT> struct GeppLds {
+    T rows[2][NT / 64][TW];
+    u64 vb[2][NT / 64];
+    int key[2][NT / 64];
 };
 
 template <typename T, int NT>
-__device__ __attribute__((always_inline)) int reg_gepp(RowH<T> &x, int nr, int nc, int *order, T *prow,
-                                                       double (*wv)[NT / 64], int (*wk)[NT / 64], int &step_of)
+__device__ __attribute__((always_inline)) int reg_gepp(RowH<T> &x, int nr, int nc, int *order, GeppLds<T, NT> &sh,
+                                                       int &step_of)
 {
     constexpr int NW = NT / 64;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -90,32 +80,38 @@ __device__ __attribute__((always_inline)) int reg_gepp(RowH<T> &x, int nr, int n
     auto step = [&](auto kc) __attribute__((always_inline)) {
         constexpr int k = decltype(kc)::value, jk = k / 2, hk = k % 2;
         if (k >= npiv) return;
-        const T mine = x.get(jk), other = shfl_xor32(mine);
-        const T a = h == hk ? mine : other;                 // column k of this row
-        double v = (live && h == hk) ? cabs1(a) : -1.0;
-        if (live && h == hk && !(v >= 0)) v = 0;           // a NaN never outranks a live row
-        int key = (live && h == hk) ? ((pos << 16) | row) : 0x7fffffff;
-        wave_argmax(v, key);
         const int b = k & 1;
-        if (lane == 0) { wv[b][w] = v; wk[b][w] = key; }
-        __syncthreads();
-        v = wv[b][0];
-        key = wk[b][0];
-#pragma unroll
-        for (int q = 1; q < NW; ++q) better_key(v, key, wv[b][q], wk[b][q]);
-        const int ps = key & 0xffff, p = key >> 16;
-        if (row == ps) {
+        const T mine = x.get(jk);
+        const T other = shfl_xor32(mine);                    // column k of this row, for the non-owning half
+        const bool cand = live && h == hk;
+        u64 vb = cand ? mag_bits(mine) : 0ull;
+        if (cand && !(vb <= (sizeof(T) == 8 ? 0x7f800000ull : 0x7ff0000000000000ull))) vb = 0;   // NaN: as zero
+        int key = cand ? ((pos << 16) | row) : 0x7fffffff;
+        wave_best(vb, key);
+        if ((key & 0xffff) == row && key != 0x7fffffff) {     // this wave's best row publishes itself
             auto pub = [&](auto jc) __attribute__((always_inline)) {
                 constexpr int j = decltype(jc)::value;
-                prow[2 * j + h] = x.get(j);
+                sh.rows[b][w][2 * j + h] = x.get(j);
             };
             Unroll<jk, TH>::run(pub);
+        }
+        if (lane == 0) { sh.vb[b][w] = vb; sh.key[b][w] = key; }
+        __syncthreads();
+        u64 gv = sh.vb[b][0];
+#pragma unroll
+        for (int q = 1; q < NW; ++q) gv = max(gv, sh.vb[b][q]);
+        int gk = 0x7fffffff;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) gk = sh.vb[b][q] == gv ? min(gk, sh.key[b][q]) : gk;
+        const int ps = gk & 0xffff, p = gk >> 16;
+        const T *prow = sh.rows[b][ps >> 5];
+        if (row == ps) {
             live = false;
             step_of = k;
         }
         if (t == 0) order[k] = ps;
-        __syncthreads();
         if (live) {
+            const T a = csel(h == hk, mine, other);
             if (pos == k) pos = p;
             const T pv = prow[k];
             const T l = (pv.x == 0 && pv.y == 0) ? a : tdiv(a, pv);
@@ -143,11 +139,9 @@ __global__ __launch_bounds__(2 * Tour<T>::RL) void k_tslu(const T *A, long lda, 
                                                          int *info)
 {
     constexpr int RL = Tour<T>::RL, AR = Tour<T>::AR, NT = 2 * RL, NW = NT / 64;
-    __shared__ T prow[TW];
+    __shared__ GeppLds<T, NT> gl;
     __shared__ int ids[RL];
     __shared__ int order[TW];
-    __shared__ double wv[2][NW];
-    __shared__ int wk[2][NW];
     __shared__ int go, counts[AR];
     const int t = threadIdx.x, lane = t & 63;
     const int row = (t >> 6) * 32 + (lane & 31), h = lane >> 5;
@@ -159,14 +153,14 @@ __global__ __launch_bounds__(2 * Tour<T>::RL) void k_tslu(const T *A, long lda, 
     {
         auto ld = [&](auto jc) __attribute__((always_inline)) {
             constexpr int j = decltype(jc)::value;
-            x.set(j, (2 * j + h < jb && row < nr) ? A[(j0 + 2 * j + h) * lda + r0 + row] : zero);
+            x.set(j, ldz(2 * j + h < jb && row < nr, A + (j0 + 2 * j + h) * lda + r0 + row));
         };
         Unroll<0, TH>::run(ld);
     }
     if (t < nr) ids[t] = (int)(r0 + t);
     __syncthreads();
     int step_of;
-    int cnt = reg_gepp<T, NT>(x, nr, jb, order, prow, wv, wk, step_of);
+    int cnt = reg_gepp<T, NT>(x, nr, jb, order, gl, step_of);
     const int li = t & 63, cg = t >> 6;          // nominee li, columns cg, cg + NW, ...
     int lsize = P, loff = 0;
     while (lsize > 1) {
@@ -185,7 +179,7 @@ __global__ __launch_bounds__(2 * Tour<T>::RL) void k_tslu(const T *A, long lda, 
 #pragma unroll
                     for (int j = 0; j < 8; ++j) {
                         const int c = c0 + j * NW;
-                        v[j] = (li < cnt && c < jb) ? A[(j0 + c) * lda + r] : zero;
+                        v[j] = ldz(li < cnt && c < jb, A + (j0 + c) * lda + r);
                     }
 #pragma unroll
                     for (int j = 0; j < 8; ++j) {
@@ -218,12 +212,12 @@ __global__ __launch_bounds__(2 * Tour<T>::RL) void k_tslu(const T *A, long lda, 
             const TNode<T> *cn = c0n + ch;
             auto ld = [&](auto jc) __attribute__((always_inline)) {
                 constexpr int j = decltype(jc)::value;
-                x.set(j, (have && 2 * j + h < jb) ? cn->v[(2 * j + h) * TW + i] : zero);
+                x.set(j, ldz(have && 2 * j + h < jb, cn->v + (2 * j + h) * TW + i));
             };
             Unroll<0, TH>::run(ld);
             if (have && h == 0) ids[row] = cn->id[i];
             __syncthreads();
-            cnt = reg_gepp<T, NT>(x, nr, jb, order, prow, wv, wk, step_of);
+            cnt = reg_gepp<T, NT>(x, nr, jb, order, gl, step_of);
         }
         // a node with one child nominates the child's rows unchanged
         loff = poff;
@@ -234,7 +228,7 @@ __global__ __launch_bounds__(2 * Tour<T>::RL) void k_tslu(const T *A, long lda, 
     if (step_of >= 0) {
         auto st = [&](auto jc) __attribute__((always_inline)) {
             constexpr int j = decltype(jc)::value;
-            top[(2 * j + h) * TW + step_of] = 2 * j + h < jb ? x.get(j) : zero;
+            top[(2 * j + h) * TW + step_of] = csel(2 * j + h < jb, x.get(j), zero);
         };
         Unroll<0, TH>::run(st);
     }

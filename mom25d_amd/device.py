@@ -270,8 +270,8 @@ class Factor:
     """An LU factor on the device: A = Z^T (column-major view of the row-major buffer) = P L U in place, ipiv
     0-based, perm its permutation; dtype complex128 or complex64."""
 
-    def __init__(self, LU, ipiv, perm, info, dinv=None):
-        self.LU, self.ipiv, self.perm, self.info, self.dinv = LU, ipiv, perm, info, dinv
+    def __init__(self, LU, ipiv, perm, info):
+        self.LU, self.ipiv, self.perm, self.info = LU, ipiv, perm, info
         self.n = LU.shape[0]
         self.dtype = LU.dtype
 
@@ -297,9 +297,7 @@ def getrf(Z, pivot='tournament'):
     pv = L.PIVOT_TOURNAMENT if pivot == 'tournament' else L.PIVOT_PARTIAL
     perm = torch.empty(n, dtype=i32, device=dv)
     L.call('m25x_getrf', dt, pv, ptr(Z), n, Z.stride(0), ptr(ipiv), ptr(perm), ptr(info), ptr(work), wb, stream())
-    dinv = torch.empty(L.lib().m25x_dinv_bytes(dt, n), dtype=torch.uint8, device=dv)
-    L.call('m25x_trtri_blocks', dt, ptr(Z), n, Z.stride(0), ptr(dinv), stream())
-    return Factor(Z, ipiv, perm, info, dinv)
+    return Factor(Z, ipiv, perm, info)
 
 
 def getrs(F, B, trans=1):
@@ -315,7 +313,7 @@ def getrs(F, B, trans=1):
         X = B[:, c0:c0 + kc].to(F.dtype).contiguous()
         wb = L.lib().m25x_getrs_work_bytes(dt, n, kc)
         work = torch.empty(wb, dtype=torch.uint8, device=B.device)
-        L.call('m25x_getrs', dt, ptr(F.LU), n, F.LU.stride(0), ptr(F.ipiv), ptr(F.perm), ptr(F.dinv), ptr(X), kc, kc,
+        L.call('m25x_getrs', dt, ptr(F.LU), n, F.LU.stride(0), ptr(F.ipiv), ptr(F.perm), ptr(X), kc, kc,
                int(trans), ptr(work), wb, stream())
         out[:, c0:c0 + kc] = X.to(c128)
     return out[:, 0] if one else out

@@ -2,8 +2,10 @@
 
 Right-preconditioned, k systems in lockstep: every operator / preconditioner call is one (N x k_active) product on
 the GPU (the FP64 Z product of m25x_zgemm_nk, the complex64 factor's solves of m25x_getrs). The Krylov basis stays
-on the device; the small Hessenberg columns come to the host once per iteration for the conjugated Givens
-rotations, the convergence test and the stall cut-off, exactly as the reference computes them.
+on the device and is orthogonalised by classical Gram-Schmidt applied twice (the reference uses modified
+Gram-Schmidt: the same basis to rounding, in four batched products per iteration instead of 2 (j + 1)); the small
+Hessenberg columns come to the host once per iteration for the conjugated Givens rotations, the convergence test
+and the stall cut-off, as the reference computes them.
 """
 import numpy as np
 import torch
@@ -47,13 +49,15 @@ def gmres_batched(A, B, M=None, x0=None, rtol=1e-6, restart=30, maxiter=10, stal
         for j in range(restart):
             Zs[j] = M(V[j])
             Wv = A(Zs[j])
-            hcol = torch.zeros((j + 2, ka), dtype=torch.complex128, device=dev)
-            for i in range(j + 1):                                   # modified Gram-Schmidt
-                hij = (V[i].conj() * Wv).sum(0)
-                hcol[i] = hij
-                Wv = Wv - V[i] * hij
+            # classical Gram-Schmidt twice (as stable as modified Gram-Schmidt, in four batched products
+            # instead of 2 (j + 1) small ones)
+            Vj = V[:j + 1]
+            h1 = (Vj.conj() * Wv[None]).sum(1)
+            Wv = Wv - (Vj * h1[:, None, :]).sum(0)
+            h2 = (Vj.conj() * Wv[None]).sum(1)
+            Wv = Wv - (Vj * h2[:, None, :]).sum(0)
             hn = torch.linalg.vector_norm(Wv, dim=0)
-            hcol[j + 1] = hn
+            hcol = torch.cat([h1 + h2, hn[None].to(torch.complex128)], 0)
             V[j + 1] = Wv / torch.where(hn > 0, hn, torch.ones_like(hn))
             hc = hcol.cpu().numpy()
             H[:j + 2, j] = hc
