@@ -517,115 +517,194 @@ __device__ __attribute__((always_inline)) T subst(T t, const T (*Ms)[SB + 1], in
     return t;
 }
 
-// Workgroup b (in publication order) owns a 64-row block: t = b_b - sum over the blocks solved before it of
-// Op_ba y_a, taken block by block as the flags come up (thread (row, slice) keeps partial sums over its 16-column
-// slice; the immediate predecessor's entries are in registers before its flag), then Op_bb y_b = t by
-// substitution in one wave, then the publish.
+// One launch of nb + 1 workgroups. Workgroup 0, the chain, solves the diagonal blocks one after another in
+// publication order: t = b_b - P_b - Op_{b, pred} y_pred (y_pred from its own LDS, Op_{b, pred} in registers, loaded
+// during the previous block), then Op_bb y_b = t by substitution (rows scaled to a unit diagonal; one wave a
+// right-hand side, lane i row i), y_b to B and its flag up. Workgroup 1 + q, a helper, accumulates for the q-th block
+// P_b = sum over the blocks published before its predecessor of Op_ba y_a as their flags come up (thread (row, slice)
+// keeps partial sums over its 16-column slice) and raises its own flag. No flag hand-off between workgroups sits
+// on the chain's path but the helpers' partial sums, which are a block ahead.
+// rdg: 1 / U_ii (kinds 0 and 3), or NULL (the unit triangles). P: nb x 64 x MAXRHS partial sums.
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsv(const T *LU, long lda, long n, T *B, long ldb, int nrhs, int kind,
-                                              int *flags, int *abort)
+__global__ __launch_bounds__(256) void k_trsv(const T *LU, long lda, long n, const T *rdg, T *B, long ldb, int nrhs,
+                                              int kind, T *P, int *yflag, int *pflag, int *abort)
 {
-    __shared__ T Ms[SB][SB + 1];             // the operator's diagonal block, Ms[i][k]
-    __shared__ T rdiag[SB];                  // 1 / its diagonal (1 for the unit triangles)
-    __shared__ T acc[SB][MAXRHS];
+    __shared__ T Ms[SB][SB + 1];
     __shared__ T ys[SB][MAXRHS];
     __shared__ T part[4][SB][MAXRHS];
     __shared__ int s_avail;
     const int nb = (int)((n + SB - 1) / SB);
     const bool fwd = kind_fwd(kind);
-    const bool unit = kind == 1 || kind == 2;
-    const int b = fwd ? blockIdx.x : nb - 1 - blockIdx.x;     // publication order is blockIdx order
-    const long i0 = (long)b * SB;
-    const int bn = (int)std::min((long)SB, n - i0);
-    const int tid = threadIdx.x, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63, row = tid & 63, kq = tid >> 6;
     const bool rowmajor = kind <= 1;
-    const T zero = CX<T>::make(0, 0);
-    const int row = tid & 63, kq = tid >> 6;                  // 4 slices of 16 columns
-    for (int idx = tid; idx < SB * SB; idx += 256) {
-        const int i = rowmajor ? idx / SB : idx % SB, k = rowmajor ? idx % SB : idx / SB;
-        Ms[i][k] = ldz(i < bn && k < bn, rowmajor ? LU + (i0 + i) * lda + i0 + k : LU + (i0 + k) * lda + i0 + i);
-    }
-    for (int idx = tid; idx < SB * MAXRHS; idx += 256) {
-        const int i = idx / MAXRHS, r = idx % MAXRHS;
-        acc[i][r] = ldz(i < bn && r < nrhs, B + (i0 + i) * ldb + r);
-    }
-    __syncthreads();
-    if (tid < SB) rdiag[tid] = (unit || tid >= bn) ? CX<T>::make(1, 0) : trecip(Ms[tid][tid]);
-    if (!unit) {                             // rows scaled by the reciprocal diagonal: a unit-diagonal substitution
-        __syncthreads();
-        for (int idx = tid; idx < SB * SB; idx += 256) {
-            const int i = idx / SB, k = idx % SB;
-            Ms[i][k] = tmul(Ms[i][k], rdiag[i]);
-        }
-    }
+    const T zero = CX<T>::make(0, 0), one = CX<T>::make(1, 0);
     auto order = [&](int q) { return fwd ? q : nb - 1 - q; };
-    auto entry = [&](int a, int u) -> T {
-        const long gi = i0 + row, gk = (long)a * SB + kq * 16 + u;
-        return ldz(row < bn && gk < n, rowmajor ? LU + gi * lda + gk : LU + gk * lda + gi);
+    auto blen = [&](int b) { return (int)std::min((long)SB, n - (long)b * SB); };
+    // the operator entry (row i of block b, column k of block a), zero outside
+    auto entry = [&](int b, int a, int i, int k) -> T {
+        const long gi = (long)b * SB + i, gk = (long)a * SB + k;
+        return ldz(gi < n && gk < n, rowmajor ? LU + gi * lda + gk : LU + gk * lda + gi);
     };
-    auto stage_y = [&](int a) {              // ys = y_a, zero padded
+    auto wait_flag = [&](int *f) {
+        long spins = 0;
+        while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > (1L << 26)) { __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
+        }
+    };
+    auto stage_y = [&](int a) {              // ys = y_a from B, zero padded
         const long k0 = (long)a * SB;
-        const int kn = (int)std::min((long)SB, n - k0);
+        const int kn = blen(a);
         for (int idx = tid; idx < SB * MAXRHS; idx += 256) {
             const int i = idx / MAXRHS, r = idx % MAXRHS;
             ys[i][r] = ldz(i < kn && r < nrhs, B + (k0 + i) * ldb + r);
         }
     };
-    auto wait_flag = [&](int a) {
-        long spins = 0;
-        while (__hip_atomic_load(flags + a, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-            __builtin_amdgcn_s_sleep(1);
-            if (++spins > (1L << 26)) { __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
+    if (blockIdx.x > 0) {
+        // ---- helper of the q-th block: the blocks published before its predecessor --------------------------
+        const int q = blockIdx.x - 1;
+        if (q < 2) return;
+        const int b = order(q), bn = blen(b);
+        T s[MAXRHS];
+#pragma unroll
+        for (int r = 0; r < MAXRHS; ++r) s[r] = zero;
+        int done = 0;
+        const int nbulk = q - 1;
+        while (done < nbulk) {
+            if (tid == 0) {
+                wait_flag(yflag + order(done));
+                int pp = done + 1;
+                while (pp < nbulk && __hip_atomic_load(yflag + order(pp), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0) ++pp;
+                s_avail = pp;
+            }
+            __syncthreads();
+            const int avail = s_avail;
+            for (int pp = done; pp < avail; ++pp) {
+                const int a = order(pp);
+                T m[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) m[u] = entry(b, a, row, kq * 16 + u);
+                stage_y(a);
+                __syncthreads();
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+#pragma unroll
+                    for (int r = 0; r < MAXRHS; ++r) s[r] = tadd(s[r], tmul(m[u], ys[kq * 16 + u][r]));
+                __syncthreads();
+            }
+            done = avail;
         }
-    };
-    const int nprev = (int)blockIdx.x;
-    T ml[16];                                 // the immediate predecessor's entries
 #pragma unroll
-    for (int u = 0; u < 16; ++u) ml[u] = nprev > 0 ? entry(order(nprev - 1), u) : CX<T>::make(0, 0);
-    T s[MAXRHS];
-#pragma unroll
-    for (int r = 0; r < MAXRHS; ++r) s[r] = zero;
-    int done = 0;
-    while (done < nprev) {
-        if (tid == 0) {
-            wait_flag(order(done));
-            int q = done + 1;
-            while (q < nprev && __hip_atomic_load(flags + order(q), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0) ++q;
-            s_avail = q;
+        for (int r = 0; r < MAXRHS; ++r) part[kq][row][r] = s[r];
+        __syncthreads();
+        T *Pb = P + (size_t)b * SB * MAXRHS;
+        for (int idx = tid; idx < bn * MAXRHS; idx += 256) {
+            const int i = idx / MAXRHS, r = idx % MAXRHS;
+            Pb[idx] = tadd(tadd(part[0][i][r], part[1][i][r]), tadd(part[2][i][r], part[3][i][r]));
         }
         __syncthreads();
-        const int avail = s_avail;
-        for (int q = done; q < avail; ++q) {
-            const int a = order(q);
-            T m[16];
-#pragma unroll
-            for (int u = 0; u < 16; ++u) m[u] = csel(q == nprev - 1, ml[u], entry(a, u));
-            stage_y(a);
-            __syncthreads();
-#pragma unroll
-            for (int u = 0; u < 16; ++u)
-#pragma unroll
-                for (int r = 0; r < MAXRHS; ++r) s[r] = tadd(s[r], tmul(m[u], ys[kq * 16 + u][r]));
-            __syncthreads();
+        if (tid == 0) {
+            __threadfence();
+            __hip_atomic_store(pflag + b, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        done = avail;
+        return;
     }
+    // ---- the chain --------------------------------------------------------------------------------------------
+    // thread (row, kq) holds entries (row, kq * 16 + u) of the next block's diagonal operator (scaled by its row's
+    // reciprocal diagonal) and of its coupling to the current block
+    // (the loads are only issued here; nothing consumes them before the next block, so they are in flight during
+    // this block's substitution)
+    T md[16], mo[16], mrd = one;
+    auto fetch_block = [&](T (&m)[16], int b, int a) {
+        const long gi = (long)b * SB + row, gk = (long)a * SB + kq * 16;
+        if ((long)(b + 1) * SB <= n && (long)(a + 1) * SB <= n) {          // a full block: no guards
+            const T *p = rowmajor ? LU + gi * lda + gk : LU + gk * lda + gi;
+            const long st = rowmajor ? 1 : lda;
 #pragma unroll
-    for (int r = 0; r < MAXRHS; ++r) part[kq][row][r] = s[r];
-    __syncthreads();
-    // t = acc - sum, then the substitution: wave w takes right-hand sides w and w + 4, lane i row i; with the rows
-    // scaled to a unit diagonal a step is one broadcast and one multiply-subtract
-    for (int r = tid >> 6; r < nrhs; r += 4) {
-        T t = tmul(tsub(acc[lane][r], tadd(tadd(part[0][lane][r], part[1][lane][r]),
-                                            tadd(part[2][lane][r], part[3][lane][r]))), rdiag[lane]);
-        t = fwd ? subst<true>(t, Ms, bn, lane) : subst<false>(t, Ms, bn, lane);
-        if (lane < bn) B[(i0 + lane) * ldb + r] = t;
+            for (int u = 0; u < 16; ++u) m[u] = p[u * st];
+        } else {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) m[u] = entry(b, a, row, kq * 16 + u);
+        }
+    };
+    auto fetch = [&](int q) {
+        const int b = order(q);
+        if (rdg) mrd = ldz((long)b * SB + row < n, rdg + (long)b * SB + row);
+        fetch_block(md, b, b);
+        if (q > 0) fetch_block(mo, b, order(q - 1));
+    };
+    const int w = tid >> 6;
+    const bool has0 = w < nrhs, has1 = w + 4 < nrhs;
+    // the right-hand side of block q less the helpers' partial sum, and the row's reciprocal diagonal, for this
+    // thread's row (lane) and right-hand sides w, w + 4: loaded a block ahead (each wave polls the helper's flag)
+    // (loads issued here, consumed by the next block's solve)
+    T bx0 = zero, bx1 = zero, px0 = zero, px1 = zero, rdn = one;
+    auto prefetch_rhs = [&](int q) {
+        const int b = order(q), bn = blen(b);
+        const long i0 = (long)b * SB;
+        const bool in = lane < bn;
+        if (has0) bx0 = ldz(in, B + (i0 + lane) * ldb + w);
+        if (has1) bx1 = ldz(in, B + (i0 + lane) * ldb + w + 4);
+        if (rdg) rdn = ldz(i0 + lane < n, rdg + i0 + lane);
+        px0 = px1 = zero;
+        if (q >= 2) {
+            if (lane == 0) wait_flag(pflag + b);
+            const T *Pb = P + (size_t)b * SB * MAXRHS + lane * MAXRHS;
+            if (has0) px0 = ldz(in, Pb + w);
+            if (has1) px1 = ldz(in, Pb + w + 4);
+        }
+    };
+    fetch(0);
+    prefetch_rhs(0);
+    for (int q = 0; q < nb; ++q) {
+        const int b = order(q), bn = blen(b);
+        const long i0 = (long)b * SB;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) Ms[row][kq * 16 + u] = tmul(md[u], mrd);
+        if (q >= 1) {                                    // the predecessor's y is in ys
+#pragma unroll
+            for (int r = 0; r < MAXRHS; ++r) {
+                if (r < nrhs) {
+                    T v = zero;
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) v = tadd(v, tmul(mo[u], ys[kq * 16 + u][r]));
+                    part[kq][row][r] = v;
+                }
+            }
+        }
+        if (q + 1 < nb) fetch(q + 1);                    // in flight during this block's substitution
+        __syncthreads();
+        auto solve = [&](int r, T v, T pv) {
+            v = tsub(v, pv);
+            if (q >= 1) v = tsub(v, tadd(tadd(part[0][lane][r], part[1][lane][r]), tadd(part[2][lane][r], part[3][lane][r])));
+            v = tmul(v, rdn);
+            return fwd ? subst<true>(v, Ms, bn, lane) : subst<false>(v, Ms, bn, lane);
+        };
+        // substitution: wave w takes right-hand sides w and w + 4
+        T t0 = zero, t1 = zero;
+        if (has0) t0 = solve(w, bx0, px0);
+        if (has1) t1 = solve(w + 4, bx1, px1);
+        __syncthreads();                                 // every read of ys, part and Ms done
+        if (has0) ys[lane][w] = t0;
+        if (has1) ys[lane][w + 4] = t1;
+        if (lane < bn) {
+            if (has0) B[(i0 + lane) * ldb + w] = t0;
+            if (has1) B[(i0 + lane) * ldb + w + 4] = t1;
+        }
+        __syncthreads();
+        // wave 3 raises the flag (its release waits on its own outstanding memory operations only) before it
+        // issues its prefetches; the other waves prefetch at once
+        if (tid == 192) __hip_atomic_store(yflag + b, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (q + 1 < nb) prefetch_rhs(q + 1);
     }
-    __syncthreads();
-    if (tid == 0) {
-        __threadfence();
-        __hip_atomic_store(flags + b, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+}
+
+// 1 / U_ii for the substitutions with U and U^T
+template <typename T>
+__global__ void k_rdiag(const T *LU, long lda, long n, T *rd)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) rd[i] = trecip(LU[i * lda + i]);
 }
 
 // out[perm[i]] = in[i] (scatter, trans) or out[i] = in[perm[i]] (gather)
@@ -760,7 +839,8 @@ int getrf_tp(T *A, long n, long lda, int *ipiv, int *perm, int *info, void *work
 template <typename T> size_t getrs_work(long n, long ldb)
 {
     const long nb = (n + SB - 1) / SB;
-    return (size_t)n * ldb * sizeof(T) + sizeof(int) * ((size_t)n + 2 * nb + 64);
+    return (size_t)n * ldb * sizeof(T) + (size_t)n * sizeof(T) + (size_t)nb * SB * MAXRHS * sizeof(T) +
+           sizeof(int) * ((size_t)n + 4 * nb + 64);
 }
 
 template <typename T>
@@ -774,25 +854,32 @@ int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, 
     const int nb = (int)((n + SB - 1) / SB);
     const size_t tmp_bytes = (size_t)n * ldb * sizeof(T);
     T *tmp = (T *)work;
-    int *perm = (int *)((char *)work + tmp_bytes);
-    int *flags = perm + n;
-    int *abort = flags + 2 * nb;
-    hipLaunchKernelGGL(k_zero_ints, dim3(blocks(2 * nb + 1, 256)), dim3(256), 0, s, flags, 2 * nb + 1);
+    T *rd = tmp + (size_t)n * ldb;
+    T *P = rd + n;
+    int *perm = (int *)(P + (size_t)nb * SB * MAXRHS);
+    int *flags = perm + n;                      // y flags and partial-sum flags of the two solves, abort
+    int *abort = flags + 4 * nb;
+    hipLaunchKernelGGL(k_zero_ints, dim3(blocks(4 * nb + 1, 256)), dim3(256), 0, s, flags, 4 * nb + 1);
+    hipLaunchKernelGGL(k_rdiag<T>, dim3(blocks(n, 256)), dim3(256), 0, s, LU, lda, n, rd);
     if (!perm_in) {
         hipLaunchKernelGGL(k_perm, dim3(1), dim3(256), 0, s, ipiv, n, perm);
         perm_in = perm;
     }
     if (int rc = check_launch("getrs")) return rc;
+    const dim3 g(nb + 1);
+    int *f0 = flags, *f1 = flags + 2 * nb;
     if (trans) {
-        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, B, ldb, nrhs, 0, flags, abort);
-        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, B, ldb, nrhs, 1, flags + nb, abort);
+        hipLaunchKernelGGL(k_trsv<T>, g, dim3(256), 0, s, LU, lda, n, (const T *)rd, B, ldb, nrhs, 0, P, f0, f0 + nb, abort);
+        hipLaunchKernelGGL(k_trsv<T>, g, dim3(256), 0, s, LU, lda, n, (const T *)nullptr, B, ldb, nrhs, 1, P, f1, f1 + nb,
+                           abort);
         hipLaunchKernelGGL(k_permute<T>, dim3(blocks(n * nrhs, 256)), dim3(256), 0, s, (const T *)B, tmp, n, nrhs, ldb,
                            perm_in, 1);
     } else {
         hipLaunchKernelGGL(k_permute<T>, dim3(blocks(n * nrhs, 256)), dim3(256), 0, s, (const T *)B, tmp, n, nrhs, ldb,
                            perm_in, 0);
-        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, tmp, ldb, nrhs, 2, flags, abort);
-        hipLaunchKernelGGL(k_trsv<T>, dim3(nb), dim3(256), 0, s, LU, lda, n, tmp, ldb, nrhs, 3, flags + nb, abort);
+        hipLaunchKernelGGL(k_trsv<T>, g, dim3(256), 0, s, LU, lda, n, (const T *)nullptr, tmp, ldb, nrhs, 2, P, f0, f0 + nb,
+                           abort);
+        hipLaunchKernelGGL(k_trsv<T>, g, dim3(256), 0, s, LU, lda, n, (const T *)rd, tmp, ldb, nrhs, 3, P, f1, f1 + nb, abort);
     }
     if (int rc = check_launch("getrs")) return rc;
     M25X_HIP(hipMemcpyAsync(B, tmp, tmp_bytes, hipMemcpyDeviceToDevice, s));
