@@ -7,10 +7,11 @@ every point).
 
 A step is one frequency point: Green's tables, fill, near field, vias, factor (fresh or reused), refinement and S;
 the timed steps walk the sweep in order from a state with no kept factor. Prints one JSON line (value: mixed
-solves/s) with the FP64 path's figures, p50 / p99 latency, the roofline of the dominant single kernel (HIP events
-around its launches, on the stream it runs on) and the CPU baseline (the oracle - the reference's solve restated,
-its fill bit-identical to the reference's compiled kernels - with SciPy's LAPACK, on this host's cores, one
-frequency of the same board).
+solves/s) with the FP64 path's figures, p50 / p99 latency, the roofline of the sweep's dominant kernels - the
+triangular solves of the refinement, HIP events on their stream around every call in the timed sweep - and of the
+far fill, and the CPU baseline (the oracle - the reference's solve restated, its fill bit-identical to the
+reference's compiled kernels - with SciPy's LAPACK, on this host's cores, one frequency of the same board).
+--roofline-only runs just those kernels (for rocprofv3).
 """
 import argparse
 import json
@@ -49,6 +50,34 @@ def sweep(sim, freqs, steps):
         lat.append(time.perf_counter() - t)
     torch.cuda.synchronize()
     return time.perf_counter() - t0, np.array(lat), S
+
+
+def solve_roofline(n, es):
+    """The triangular solves (k_trsv: U^T, then L^T, in each m25x_getrs call), the sweep's dominant kernel: the
+    library's HIP events on the launching stream around every launch recorded since dv.trsv_timing(True).
+    Algorithmic bytes per launch = its triangle of the factor (n (n +- 1) / 2 elements) and the right-hand sides
+    read and written once."""
+    from mom25d_amd import device as dv
+    ms, by = dv.trsv_times()
+    # the launches on a factor of element size es (the sweep's FP64 fall-backs read twice the bytes)
+    lim = (n * (n + 1) / 2 + 16.0 * n) * 8
+    sel = by <= lim if es == 8 else by > lim
+    ms, by = ms[sel], by[sel]
+    avg = float(ms.mean())
+    return dict(kernel=f"k_trsv<{'float2' if es == 8 else 'double2'}> (m25x_getrs, N={n}, up to 4 right-hand sides)",
+                ms=avg, calls=int(sel.sum()), bytes=float(by.mean()), achieved=float(by.mean() / (avg * 1e-3)))
+
+
+def measured_traffic(kernel):
+    """HBM bytes a dispatch of `kernel` moved, as recorded in profiles/trsv_fill_board_n7799_hbm_traffic.txt
+    (rocprofv3 FETCH_SIZE and WRITE_SIZE of `bench.py --roofline-only`, scaled by a 1 GiB copy), or None."""
+    try:
+        for line in open(os.path.join(ROOT, 'profiles', 'trsv_fill_board_n7799_hbm_traffic.txt')):
+            if line.startswith(kernel + ' ') and 'MB a dispatch' in line:
+                return float(line.rsplit('=', 1)[1].split('MB')[0]) * 1e6
+    except OSError:
+        pass
+    return None
 
 
 def kernel_roofline(sim, f, reps=5):
@@ -117,6 +146,39 @@ def cpu_baseline(sim, f):
                        f"per-board geometry not counted)")
 
 
+def roofline_only(sim, freqs):
+    """The kernels whose rooflines the benchmark reports, alone, for rocprofv3 runs: ten m25x_getrs calls (four
+    right-hand sides) with the complex64 factor of the board at 2 GHz, ten far fills, and a 1 GiB device-to-device
+    copy whose counter readings give the scale of FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    from mom25d_amd import device as dv
+    f = freqs[5]
+    Z = sim.fast.fill(f)
+    d = dv.diag_scales(Z)
+    n = Z.shape[0]
+    Z64 = torch.empty((n, n), dtype=dv.c64, device=Z.device)
+    dv.scale_c64(Z, d, Z64)
+    F = dv.getrf(Z64)
+    B = torch.from_numpy(np.asarray(sim.excitation(), complex)).to(Z.device)
+    dv.trsv_timing(True)
+    for _ in range(10):
+        dv.getrs(F, B)
+    dv.trsv_timing(False)
+    sr = solve_roofline(n, 8)
+    del F, Z64
+    rf = kernel_roofline(sim, f, reps=10)
+    src = torch.empty(1 << 30, dtype=torch.uint8, device=Z.device)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps(dict(getrs_ms=sr['ms'], getrs_bytes=sr['bytes'], fill_ms=rf['ms'], fill_bytes=rf['bytes'],
+                          copy_bytes=1 << 30)))
+
+
+def note(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -124,19 +186,31 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-double', action='store_true')
+    ap.add_argument('--roofline-only', action='store_true',
+                    help="only the roofline kernels, for profilers: the complex64 factor of the board at 2 GHz, ten "
+                         "solves with 4 right-hand sides, ten far fills, and a 1 GiB device copy (the counters' scale)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU: frequency points and boards are independent "
-                                   "and would shard across GPUs with no exchange (see DESIGN.md); --gpus 1"}))
+                                   "and would shard across GPUs with no exchange; --gpus 1"}))
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
     freqs = np.geomspace(1e9, 8e9, 16)
     sm = build_sim('mixed')
     N = sm.Ne + sm.Nv
+    note(f"board built, N = {N}")
+    if a.roofline_only:
+        roofline_only(sm, freqs)
+        return
+    from mom25d_amd import device as dv
     for i in range(a.warmup):
         sm.sparams(freqs[i % len(freqs)])
+    note(f"{a.warmup} warm-up points done")
+    dv.trsv_timing(True)
     T, lat, Sm = sweep(sm, freqs, a.steps)
+    dv.trsv_timing(False)
+    note(f"timed mixed sweep: {a.steps} points in {T:.3f} s")
     value = a.steps / T
     out = dict(metric="freq-point solves/s at N=8k (mixed-precision vs FP64 plain); p50/p99 solve latency",
                value=round(value, 3), unit="solves/s", n_gpus=1, steps=a.steps, warmup=a.warmup,
@@ -151,24 +225,33 @@ def main():
             sd.sparams(freqs[i])
         nd = min(a.steps, 8)
         Td, latd, Sd = sweep(sd, freqs, nd)
+        note(f"FP64 sweep: {nd} points in {Td:.3f} s")
         out['fp64_plain'] = dict(value=round(nd / Td, 3), steps=nd, ms_per_step=round(1e3 * Td / nd, 3),
                                  p50_ms=round(1e3 * float(np.percentile(latd, 50)), 3),
                                  p99_ms=round(1e3 * float(np.percentile(latd, 99)), 3))
         out['mixed_over_fp64'] = round(value / (nd / Td), 3)
         out['max_dS_mixed_vs_fp64'] = float(max(np.abs(Sm[i] - Sd[i]).max() for i in range(nd)))
     lu = lu_rate(sm, freqs[5])
+    note("LU rates done")
     out['lu'] = {k: dict(ms=round(v['ms'], 3), tflops=round(v['tflops'], 2),
                          frac=round(v['tflops'] * 1e12 / (FP32_PEAK if k == 'c64' else FP64_PEAK), 4))
                  for k, v in lu.items()}
+    sr = solve_roofline(N, 8)
+    out['roofline'] = dict(bound='hbm', kernel=sr['kernel'], achieved=round(sr['achieved'] / 1e9, 1),
+                           peak=HBM_PEAK / 1e9, unit='GB/s', frac=round(sr['achieved'] / HBM_PEAK, 4),
+                           traffic=measured_traffic('k_trsv<float2>'), ms=round(sr['ms'], 4), calls=sr['calls'],
+                           algorithmic_bytes=sr['bytes'])
     rf = kernel_roofline(sm, freqs[5])
-    out['roofline'] = dict(bound='hbm', kernel=rf['kernel'], achieved=round(rf['achieved'] / 1e9, 1),
-                           peak=HBM_PEAK / 1e9, unit='GB/s', frac=round(rf['achieved'] / HBM_PEAK, 4),
-                           traffic=None, ms=round(rf['ms'], 4), algorithmic_bytes=rf['bytes'])
+    out['roofline_fill'] = dict(bound='hbm', kernel=rf['kernel'], achieved=round(rf['achieved'] / 1e9, 1),
+                                peak=HBM_PEAK / 1e9, unit='GB/s', frac=round(rf['achieved'] / HBM_PEAK, 4),
+                                traffic=measured_traffic('k_fill_far<double2>'), ms=round(rf['ms'], 4),
+                                algorithmic_bytes=rf['bytes'])
     out['fill_pairs_per_s'] = round(sm.Ne * (sm.Ne + 1) / 2 / (rf['ms'] * 1e-3), 1)
     out['config'] = dict(workload=f"C2: seeded 4-layer board (seed 2), N={N} ({sm.Ne} RWG edges + {sm.Nv} via "
                                   f"chains), 16 frequencies geomspace(1, 8 GHz) swept in order, mixed precision",
                          N=int(N), board_seed=2, freqs_ghz=[round(f / 1e9, 4) for f in freqs], near_pairs=sm.fast.plan.n)
     if not a.no_cpu:
+        note("CPU baseline (oracle, one frequency)")
         out['cpu_baseline'] = cpu_baseline(sm, freqs[5])
     print(json.dumps(out))
 

@@ -164,9 +164,18 @@ size_t m25x_getrs_work_bytes(int dtype, long n, long ldb);
 int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B, long ldb,
                int nrhs, int trans, void *work, size_t work_bytes, void *stream);
 
+/* Launch timing of the triangular solves, for benchmarks (not thread-safe). m25x_trsv_timing(1) starts a record
+ * (clearing the last): from then on each triangular-solve kernel that m25x_getrs launches is bracketed by two HIP
+ * events on its stream; m25x_trsv_timing(0) stops recording. m25x_trsv_times waits for the last recorded launch
+ * and writes, for up to cap launches, its duration (ms) and the bytes it reads and writes by algorithm (its
+ * triangle of the factor, the right-hand sides once each way); it returns the number of launches recorded, or a
+ * negative error code. No counterpart in the reference: its solve is a LAPACK / cuSOLVER call. */
+int m25x_trsv_timing(int on);
+long m25x_trsv_times(float *ms, double *bytes, long cap);
+
 /* ---- refinement products ------------------------------------------------------------------------------- */
 
-/* Y = Z X: Z n x n complex128 row-major (leading dimension ld), X and Y n x k complex128 row-major (k <= 16).
+/* Y = Z X: Z n x n complex128 row-major (leading dimension ld), X and Y n x k complex128 row-major (1 <= k <= 8).
  * The FP64 products of the GMRES refinement (src/mom25d/fast.py:1004-1007, A = Z @ V). */
 int m25x_zgemm_nk(const double *Z, long n, long ld, const double *X, int k, double *Y, void *stream);
 

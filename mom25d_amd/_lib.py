@@ -50,6 +50,8 @@ _SIGS = {
     'm25x_ipiv_perm': ([P, L, P, P], I),
     'm25x_getrs_work_bytes': ([I, L, L], S),
     'm25x_getrs': ([I, P, L, L, P, P, P, L, I, I, P, S, P], I),
+    'm25x_trsv_timing': ([I], I),
+    'm25x_trsv_times': ([P, P, L], L),
     'm25x_zgemm_nk': ([P, L, L, P, I, P, P], I),
 }
 

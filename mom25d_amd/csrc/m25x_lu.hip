@@ -467,7 +467,7 @@ __global__ __launch_bounds__(256) void k_apply_moves(T *A, long lda, long c0, lo
 
 // kind 0: U^T y = b (forward); 1: L^T z = y (backward); 2: L y = b (forward); 3: U x = y (backward).
 // Operator entry (row i, column k): kinds 0, 1 read LU[i * lda + k] (U[k][i], L[k][i]); kinds 2, 3 LU[k * lda + i].
-__device__ inline bool kind_fwd(int kind) { return kind == 0 || kind == 2; }
+__host__ __device__ inline bool kind_fwd(int kind) { return kind == 0 || kind == 2; }
 
 __device__ inline float2 readlane_c(float2 v, int l)
 {
@@ -492,17 +492,18 @@ __device__ inline double2 cfms(double2 t, double2 m, double2 y)
     return make_double2(fma(-m.x, y.x, fma(m.y, y.y, t.x)), fma(-m.y, y.x, fma(-m.x, y.y, t.y)));
 }
 
-// Unit-diagonal substitution of one right-hand side held a row a lane: step k broadcasts row k's value and the rows
-// after it (FWD) or before it subtract their multiple; the operator's row comes from LDS eight steps ahead.
+// Unit-diagonal substitution of one right-hand side held a row a lane (t already scaled by the lane's reciprocal
+// diagonal rs): step k broadcasts row k's value and the rows after it (FWD) or before it subtract their multiple; the
+// operator's row comes from LDS eight steps ahead, scaled by rs as it arrives.
 template <bool FWD, typename T>
-__device__ __attribute__((always_inline)) T subst(T t, const T (*Ms)[SB + 1], int bn, int lane)
+__device__ __attribute__((always_inline)) T subst(T t, const T (*Ms)[SB + 1], int bn, int lane, T rs)
 {
     for (int s0 = 0; s0 < bn; s0 += 8) {
         T mk[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int step = s0 + j, k = FWD ? step : bn - 1 - step;
-            mk[j] = ldz(step < bn, &Ms[lane][k < 0 ? 0 : k]);
+            mk[j] = tmul(ldz(step < bn, &Ms[lane][k < 0 ? 0 : k]), rs);
         }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
@@ -529,10 +530,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_trsv(const T *LU, long lda, long n, const T *rdg, T *B, long ldb, int nrhs,
                                               int kind, T *P, int *yflag, int *pflag, int *abort)
 {
-    __shared__ T Ms[SB][SB + 1];
+    __shared__ T Ms[SB][SB + 1];             // chain: the block's diagonal operator
+    __shared__ T Mo[SB][SB + 1];             // chain: its coupling to the predecessor; helpers: partial sums
     __shared__ T ys[SB][MAXRHS];
-    __shared__ T part[4][SB][MAXRHS];
     __shared__ int s_avail;
+    T (*part)[SB][MAXRHS] = reinterpret_cast<T (*)[SB][MAXRHS]>(&Mo[0][0]);
     const int nb = (int)((n + SB - 1) / SB);
     const bool fwd = kind_fwd(kind);
     const int tid = threadIdx.x, lane = tid & 63, row = tid & 63, kq = tid >> 6;
@@ -610,36 +612,40 @@ __global__ __launch_bounds__(256) void k_trsv(const T *LU, long lda, long n, con
         return;
     }
     // ---- the chain --------------------------------------------------------------------------------------------
-    // thread (row, kq) holds entries (row, kq * 16 + u) of the next block's diagonal operator (scaled by its row's
-    // reciprocal diagonal) and of its coupling to the current block
-    // (the loads are only issued here; nothing consumes them before the next block, so they are in flight during
-    // this block's substitution)
-    T md[16], mo[16], mrd = one;
-    auto fetch_block = [&](T (&m)[16], int b, int a) {
-        const long gi = (long)b * SB + row, gk = (long)a * SB + kq * 16;
+    // The next block's diagonal operator and coupling, loaded a block ahead with coalesced reads: thread t holds
+    // entries (rows 16 (t / 64) + u, column t % 64) of a row-major operator, (row t % 64, columns 16 (t / 64) + u) of
+    // a column-major one.
+    T md[16], mo[16];
+    auto ij = [&](int u, int &i, int &k) __attribute__((always_inline)) {
+        if (rowmajor) { i = kq * 16 + u; k = lane; }
+        else { i = lane; k = kq * 16 + u; }
+    };
+    auto fetch_block = [&](T (&m)[16], int b, int a) __attribute__((always_inline)) {
         if ((long)(b + 1) * SB <= n && (long)(a + 1) * SB <= n) {          // a full block: no guards
-            const T *p = rowmajor ? LU + gi * lda + gk : LU + gk * lda + gi;
-            const long st = rowmajor ? 1 : lda;
+            const long i0 = (long)b * SB, k0 = (long)a * SB;
+            const T *p = rowmajor ? LU + (i0 + kq * 16) * lda + k0 + lane : LU + (k0 + kq * 16) * lda + i0 + lane;
 #pragma unroll
-            for (int u = 0; u < 16; ++u) m[u] = p[u * st];
+            for (int u = 0; u < 16; ++u) m[u] = p[u * lda];
         } else {
 #pragma unroll
-            for (int u = 0; u < 16; ++u) m[u] = entry(b, a, row, kq * 16 + u);
+            for (int u = 0; u < 16; ++u) {
+                int i, k;
+                ij(u, i, k);
+                m[u] = entry(b, a, i, k);
+            }
         }
     };
-    auto fetch = [&](int q) {
+    auto fetch = [&](int q) __attribute__((always_inline)) {
         const int b = order(q);
-        if (rdg) mrd = ldz((long)b * SB + row < n, rdg + (long)b * SB + row);
         fetch_block(md, b, b);
         if (q > 0) fetch_block(mo, b, order(q - 1));
     };
     const int w = tid >> 6;
     const bool has0 = w < nrhs, has1 = w + 4 < nrhs;
     // the right-hand side of block q less the helpers' partial sum, and the row's reciprocal diagonal, for this
-    // thread's row (lane) and right-hand sides w, w + 4: loaded a block ahead (each wave polls the helper's flag)
-    // (loads issued here, consumed by the next block's solve)
+    // thread's row (lane) and right-hand sides w, w + 4: loads issued a block ahead, consumed by its solve
     T bx0 = zero, bx1 = zero, px0 = zero, px1 = zero, rdn = one;
-    auto prefetch_rhs = [&](int q) {
+    auto prefetch_rhs = [&](int q) __attribute__((always_inline)) {
         const int b = order(q), bn = blen(b);
         const long i0 = (long)b * SB;
         const bool in = lane < bn;
@@ -660,31 +666,35 @@ __global__ __launch_bounds__(256) void k_trsv(const T *LU, long lda, long n, con
         const int b = order(q), bn = blen(b);
         const long i0 = (long)b * SB;
 #pragma unroll
-        for (int u = 0; u < 16; ++u) Ms[row][kq * 16 + u] = tmul(md[u], mrd);
-        if (q >= 1) {                                    // the predecessor's y is in ys
-#pragma unroll
-            for (int r = 0; r < MAXRHS; ++r) {
-                if (r < nrhs) {
-                    T v = zero;
-#pragma unroll
-                    for (int u = 0; u < 16; ++u) v = tadd(v, tmul(mo[u], ys[kq * 16 + u][r]));
-                    part[kq][row][r] = v;
-                }
-            }
+        for (int u = 0; u < 16; ++u) {
+            int i, k;
+            ij(u, i, k);
+            Ms[i][k] = md[u];
+            if (q >= 1) Mo[i][k] = mo[u];
         }
         if (q + 1 < nb) fetch(q + 1);                    // in flight during this block's substitution
         __syncthreads();
-        auto solve = [&](int r, T v, T pv) {
+        auto solve = [&](int r, T v, T pv) __attribute__((always_inline)) {
             v = tsub(v, pv);
-            if (q >= 1) v = tsub(v, tadd(tadd(part[0][lane][r], part[1][lane][r]), tadd(part[2][lane][r], part[3][lane][r])));
+            if (q >= 1) {                                // the predecessor's y is in ys: sixteen loads in flight
+                T sa[4] = {zero, zero, zero, zero};
+                for (int k0 = 0; k0 < SB; k0 += 16) {
+                    T mv[16], yv[16];
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) { mv[j] = Mo[lane][k0 + j]; yv[j] = ys[k0 + j][r]; }
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) sa[j & 3] = tadd(sa[j & 3], tmul(mv[j], yv[j]));
+                }
+                v = tsub(v, tadd(tadd(sa[0], sa[1]), tadd(sa[2], sa[3])));
+            }
             v = tmul(v, rdn);
-            return fwd ? subst<true>(v, Ms, bn, lane) : subst<false>(v, Ms, bn, lane);
+            return fwd ? subst<true>(v, Ms, bn, lane, rdn) : subst<false>(v, Ms, bn, lane, rdn);
         };
-        // substitution: wave w takes right-hand sides w and w + 4
+        // wave w takes right-hand sides w and w + 4, lane i row i
         T t0 = zero, t1 = zero;
         if (has0) t0 = solve(w, bx0, px0);
         if (has1) t1 = solve(w + 4, bx1, px1);
-        __syncthreads();                                 // every read of ys, part and Ms done
+        __syncthreads();                                 // every read of ys, Mo and Ms done
         if (has0) ys[lane][w] = t0;
         if (has1) ys[lane][w + 4] = t1;
         if (lane < bn) {
@@ -843,6 +853,44 @@ template <typename T> size_t getrs_work(long n, long ldb)
            sizeof(int) * ((size_t)n + 4 * nb + 64);
 }
 
+// ---- launch timing of the triangular solves (benchmarks; one host thread) -----------------------------------
+// While on, every k_trsv launch is bracketed by two HIP events recorded on its stream, with the bytes it reads by
+// algorithm (its triangle of the factor, and the right-hand sides read and written once).
+struct TrsvTiming {
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<double> bytes;
+    size_t used = 0;
+};
+TrsvTiming g_tt;
+
+int tt_mark(hipStream_t s)
+{
+    if (!g_tt.on) return M25X_OK;
+    if (g_tt.used == g_tt.ev.size()) {
+        hipEvent_t e;
+        M25X_HIP(hipEventCreate(&e));
+        g_tt.ev.push_back(e);
+    }
+    M25X_HIP(hipEventRecord(g_tt.ev[g_tt.used++], s));
+    return M25X_OK;
+}
+
+template <typename T>
+int trsv_launch(hipStream_t s, int nb, const T *LU, long lda, long n, const T *rd, T *B, long ldb, int nrhs, int kind,
+                T *P, int *yflag, int *pflag, int *abort)
+{
+    if (int rc = tt_mark(s)) return rc;
+    hipLaunchKernelGGL(k_trsv<T>, dim3(nb + 1), dim3(256), 0, s, LU, lda, n, rd, B, ldb, nrhs, kind, P, yflag, pflag,
+                       abort);
+    if (int rc = check_launch("getrs/trsv")) return rc;
+    if (g_tt.on) {
+        const double tri = kind_fwd(kind) ? 0.5 * n * (n - 1) : 0.5 * n * (n + 1);
+        g_tt.bytes.push_back((tri + 2.0 * n * nrhs) * sizeof(T));
+    }
+    return tt_mark(s);
+}
+
 template <typename T>
 int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, T *B, long ldb, int nrhs, int trans,
             void *work, size_t wbytes, hipStream_t s)
@@ -866,20 +914,18 @@ int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, 
         perm_in = perm;
     }
     if (int rc = check_launch("getrs")) return rc;
-    const dim3 g(nb + 1);
     int *f0 = flags, *f1 = flags + 2 * nb;
+    int rc;
     if (trans) {
-        hipLaunchKernelGGL(k_trsv<T>, g, dim3(256), 0, s, LU, lda, n, (const T *)rd, B, ldb, nrhs, 0, P, f0, f0 + nb, abort);
-        hipLaunchKernelGGL(k_trsv<T>, g, dim3(256), 0, s, LU, lda, n, (const T *)nullptr, B, ldb, nrhs, 1, P, f1, f1 + nb,
-                           abort);
+        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, rd, B, ldb, nrhs, 0, P, f0, f0 + nb, abort))) return rc;
+        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, nullptr, B, ldb, nrhs, 1, P, f1, f1 + nb, abort))) return rc;
         hipLaunchKernelGGL(k_permute<T>, dim3(blocks(n * nrhs, 256)), dim3(256), 0, s, (const T *)B, tmp, n, nrhs, ldb,
                            perm_in, 1);
     } else {
         hipLaunchKernelGGL(k_permute<T>, dim3(blocks(n * nrhs, 256)), dim3(256), 0, s, (const T *)B, tmp, n, nrhs, ldb,
                            perm_in, 0);
-        hipLaunchKernelGGL(k_trsv<T>, g, dim3(256), 0, s, LU, lda, n, (const T *)nullptr, tmp, ldb, nrhs, 2, P, f0, f0 + nb,
-                           abort);
-        hipLaunchKernelGGL(k_trsv<T>, g, dim3(256), 0, s, LU, lda, n, (const T *)rd, tmp, ldb, nrhs, 3, P, f1, f1 + nb, abort);
+        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, nullptr, tmp, ldb, nrhs, 2, P, f0, f0 + nb, abort))) return rc;
+        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, rd, tmp, ldb, nrhs, 3, P, f1, f1 + nb, abort))) return rc;
     }
     if (int rc = check_launch("getrs")) return rc;
     M25X_HIP(hipMemcpyAsync(B, tmp, tmp_bytes, hipMemcpyDeviceToDevice, s));
@@ -932,6 +978,33 @@ int m25x_ipiv_perm(const int *ipiv, long n, int *perm, void *stream)
     if (n == 0) return M25X_OK;
     hipLaunchKernelGGL(k_perm, dim3(1), dim3(256), 0, (hipStream_t)stream, ipiv, n, perm);
     return check_launch("ipiv_perm");
+}
+
+int m25x_trsv_timing(int on)
+{
+    if (on) {
+        g_tt.used = 0;
+        g_tt.bytes.clear();
+    }
+    g_tt.on = on != 0;
+    return M25X_OK;
+}
+
+long m25x_trsv_times(float *ms, double *bytes, long cap)
+{
+    const long n = (long)g_tt.bytes.size();
+    if (n && 2 * (size_t)n <= g_tt.used) {
+        hipError_t e = hipEventSynchronize(g_tt.ev[2 * n - 1]);
+        if (e != hipSuccess) return fail(M25X_EHIP, "trsv_times: %s", hipGetErrorString(e));
+    }
+    for (long i = 0; i < n && i < cap; ++i) {
+        if (ms) {
+            hipError_t e = hipEventElapsedTime(ms + i, g_tt.ev[2 * i], g_tt.ev[2 * i + 1]);
+            if (e != hipSuccess) return fail(M25X_EHIP, "trsv_times: %s", hipGetErrorString(e));
+        }
+        if (bytes) bytes[i] = g_tt.bytes[i];
+    }
+    return n;
 }
 
 int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B, long ldb,

@@ -319,6 +319,25 @@ def getrs(F, B, trans=1):
     return out[:, 0] if one else out
 
 
+def trsv_timing(on):
+    """Start (clearing the last record) or stop timing the triangular-solve launches of getrs: HIP events on the
+    stream each launch runs on."""
+    L.call('m25x_trsv_timing', int(bool(on)))
+
+
+def trsv_times():
+    """(ms, bytes) per triangular-solve launch recorded since trsv_timing(True): durations from the events, and the
+    bytes each launch reads and writes by algorithm (its triangle of the factor, the right-hand sides)."""
+    lib = L.lib()
+    n = lib.m25x_trsv_times(None, None, 0)
+    if n < 0:
+        raise RuntimeError(lib.m25x_last_error().decode())
+    ms, by = np.zeros(n, np.float32), np.zeros(n, np.float64)
+    if n:
+        lib.m25x_trsv_times(ms.ctypes.data_as(C.c_void_p), by.ctypes.data_as(C.c_void_p), n)
+    return ms.astype(np.float64), by
+
+
 def zgemm_nk(Z, X):
     """Y = Z X with Z (n x n complex128, row-major) and X (n,) or (n, k); columns 8 at a time."""
     one = X.dim() == 1
