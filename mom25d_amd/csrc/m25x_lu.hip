@@ -492,220 +492,416 @@ __device__ inline double2 cfms(double2 t, double2 m, double2 y)
     return make_double2(fma(-m.x, y.x, fma(m.y, y.y, t.x)), fma(-m.y, y.x, fma(-m.x, y.y, t.y)));
 }
 
-// Unit-diagonal substitution of one right-hand side held a row a lane (t already scaled by the lane's reciprocal
-// diagonal rs): step k broadcasts row k's value and the rows after it (FWD) or before it subtract their multiple; the
-// operator's row comes from LDS eight steps ahead, scaled by rs as it arrives.
-template <bool FWD, typename T>
-__device__ __attribute__((always_inline)) T subst(T t, const T (*Ms)[SB + 1], int bn, int lane, T rs)
+// ---- values that carry their own ready mark ----------------------------------------------------------------
+// The solves hand results between workgroups through buffers filled beforehand with all-ones 64-bit words (a NaN
+// pattern no arithmetic produces): every 64-bit word is written and read with single-copy atomic accesses at
+// device scope, so a reader that finds no all-ones word in a value has the value itself, with no flag and no
+// second round trip. A computed word that happens to be all ones is written as the canonical NaN instead.
+typedef unsigned long long u64;
+constexpr u64 UNSET = ~0ull;
+
+__device__ inline u64 mark_safe(u64 w, u64 nan) { return w == UNSET ? nan : w; }
+__device__ inline void put(float2 *p, float2 v)
 {
-    for (int s0 = 0; s0 < bn; s0 += 8) {
-        T mk[8];
+    const u64 w = ((u64)__float_as_uint(v.y) << 32) | __float_as_uint(v.x);
+    __hip_atomic_store((u64 *)p, mark_safe(w, 0x7fc000007fc00000ull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void put(double2 *p, double2 v)
+{
+    u64 *q = (u64 *)p;
+    __hip_atomic_store(q, mark_safe((u64)__double_as_longlong(v.x), 0x7ff8000000000000ull), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, mark_safe((u64)__double_as_longlong(v.y), 0x7ff8000000000000ull), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+// the value at p (all-ones words if it has not been written yet), and whether it has been written
+__device__ inline float2 fetch(const float2 *p)
+{
+    const u64 w = __hip_atomic_load((u64 *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return make_float2(__uint_as_float((unsigned)w), __uint_as_float((unsigned)(w >> 32)));
+}
+__device__ inline double2 fetch(const double2 *p)
+{
+    const u64 a = __hip_atomic_load((u64 *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const u64 b = __hip_atomic_load((u64 *)p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return make_double2(__longlong_as_double((long long)a), __longlong_as_double((long long)b));
+}
+__device__ inline bool is_set(float2 v) { return (__float_as_uint(v.x) & __float_as_uint(v.y)) != ~0u; }
+__device__ inline bool is_set(double2 v)
+{
+    return (u64)__double_as_longlong(v.x) != UNSET && (u64)__double_as_longlong(v.y) != UNSET;
+}
+template <typename T> __device__ inline bool take(const T *p, T &v)
+{
+    v = fetch(p);
+    return is_set(v);
+}
+__device__ inline bool wave_all(bool c) { return __builtin_amdgcn_ballot_w64(!c) == 0; }
+
+// ---- the triangular solve ----------------------------------------------------------------------------------
+constexpr int TB = 512;                  // threads of a solve workgroup: 8 waves
+constexpr int NWV = TB / 64;
+constexpr int TE = SB * SB / TB;         // entries of a 64 x 64 tile a helper thread takes: 8
+constexpr int TL = 2 * TE;               // entries of a tile a loader thread stages: 16
+constexpr int HD = 2;                    // the chain applies y_q to blocks q + 1 and q + 2; helpers the rest
+constexpr int TRSV_LEAD = 3;             // blocks ahead of the chain its prefetchers read
+constexpr int TRSV_NPF = 4;              // prefetch workgroups
+template <typename T> struct TrsvMaxRhs { static constexpr int NR = 8; };    // right-hand sides a launch takes
+template <> struct TrsvMaxRhs<double2> { static constexpr int NR = 4; };
+
+// Unit-lower (FWD) or unit-upper substitution of NW right-hand sides, lane i row i, with the block's operator Ms
+// in LDS (rows pre-scaled to a unit diagonal; the diagonal, the other triangle and the padding zero): step k
+// broadcasts y_k and every lane subtracts its multiple, the zeros leaving the finished rows alone. The same
+// broadcast feeds the next block: a -= Cs[lane][k] y_k.
+template <bool FWD, int NW, typename T>
+__device__ __attribute__((always_inline)) void subst_fused(T (&t)[NW], T (&a)[NW], const T (*Ms)[SB + 1],
+                                                          const T (*Cs)[SB + 1], int lane)
+{
+#pragma unroll
+    for (int s0 = 0; s0 < SB; s0 += 8) {
+        T mk[8], mc[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int step = s0 + j, k = FWD ? step : bn - 1 - step;
-            mk[j] = tmul(ldz(step < bn, &Ms[lane][k < 0 ? 0 : k]), rs);
+            const int k = FWD ? s0 + j : SB - 1 - s0 - j;
+            mk[j] = Ms[lane][k];
+            mc[j] = Cs[lane][k];
         }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int step = s0 + j;
-            if (step < bn) {
-                const int k = FWD ? step : bn - 1 - step;
-                const T y = readlane_c(t, k);
-                t = csel(FWD ? lane > k : lane < k, cfms(t, mk[j], y), t);
+            const int k = FWD ? s0 + j : SB - 1 - s0 - j;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const T y = readlane_c(t[w], k);
+                t[w] = cfms(t[w], mk[j], y);
+                a[w] = cfms(a[w], mc[j], y);
             }
         }
     }
-    return t;
 }
 
-// One launch of nb + 1 workgroups. Workgroup 0, the chain, solves the diagonal blocks one after another in
-// publication order: t = b_b - P_b - Op_{b, pred} y_pred (y_pred from its own LDS, Op_{b, pred} in registers, loaded
-// during the previous block), then Op_bb y_b = t by substitution (rows scaled to a unit diagonal; one wave a
-// right-hand side, lane i row i), y_b to B and its flag up. Workgroup 1 + q, a helper, accumulates for the q-th block
-// P_b = sum over the blocks published before its predecessor of Op_ba y_a as their flags come up (thread (row, slice)
-// keeps partial sums over its 16-column slice) and raises its own flag. No flag hand-off between workgroups sits
-// on the chain's path but the helpers' partial sums, which are a block ahead.
-// rdg: 1 / U_ii (kinds 0 and 3), or NULL (the unit triangles). P: nb x 64 x MAXRHS partial sums.
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsv(const T *LU, long lda, long n, const T *rdg, T *B, long ldb, int nrhs,
-                                              int kind, T *P, int *yflag, int *pflag, int *abort)
+// One launch of nb + 1 workgroups of 512 threads solves one triangle for nrhs <= NR right-hand sides.
+//
+// Workgroup 0, the chain, solves the 64-row diagonal blocks in order, in three barriers a block (A, B, C).
+// Waves 0-3, its solvers (wave w takes right-hand sides w and, for NR = 8, w + 4; lane i row i): t = src_q - P_q
+// + a_q, then the unit-diagonal substitution, which also applies y_q to block q + 1 (a_{q+1}); y_q published to dst
+// and left in LDS. Waves 4-7, its loaders (thread: row lane, sixteen columns): at A they write block q's tiles
+// (the diagonal block's strict triangle, scaled; the coupling to block q + 1) from registers into LDS; after B
+// they issue the loads of block q + 1's tiles and of Op_{q+2, q}; after C, with y_q in LDS, they apply y_q to
+// block q + 2 and leave their partial sums for the solvers. No wait on device memory sits on the solvers' path but
+// for P_q, which they ask for a block ahead.
+//
+// Workgroup 1 + q, a helper, accumulates for block q the products with the blocks published before q - 2,
+// P_q = sum_a Op_qa y_a, as each y_a appears in dst, and publishes P_q: two blocks' time to do it. y (dst) and P
+// travel as self-marking values (put / take): a reader spins on the value itself.
+//
+// rdg: 1 / U_ii (read for the kinds with U only: L is unit). src: n x nrhs (leading dimension lds); dst: n x nrhs
+// (leading dimension nrhs) and P (nb x 64 x nrhs) filled with UNSET words beforehand. abort: raised if a wait
+// exceeds its bound (the result is then garbage, not a hang). NR: 4 (nrhs <= 4) or 8.
+template <typename T, int NR>
+__global__ __launch_bounds__(TB) void k_trsv(const T *LU, long lda, long n, const T *rdg, const T *src, long lds,
+                                             T *dst, int nrhs, int kind, T *P, int *abort)
 {
-    __shared__ T Ms[SB][SB + 1];             // chain: the block's diagonal operator
-    __shared__ T Mo[SB][SB + 1];             // chain: its coupling to the predecessor; helpers: partial sums
-    __shared__ T ys[SB][MAXRHS];
-    __shared__ int s_avail;
-    T (*part)[SB][MAXRHS] = reinterpret_cast<T (*)[SB][MAXRHS]>(&Mo[0][0]);
+    constexpr int NW = NR / 4;               // right-hand sides a solver wave takes
+    __shared__ T Ms[SB][SB + 1];             // chain: the diagonal operator; helpers: the slices' partial sums
+    __shared__ T Cs[SB][SB + 1];             // chain: Op_{q+1, q}; helpers: y_a by slice
+    __shared__ T rds[SB];                    // chain: 1 / U_ii of the block's rows (one for the unit triangles)
+    __shared__ T ys[NR][SB];                 // chain: y_q
+    __shared__ T lp[4][NR][SB];              // chain: the loaders' partial sums Op_{q+2, q} y_q
+    static_assert(NWV * NR * SB <= SB * (SB + 1) && NWV * 64 <= SB * (SB + 1), "helper arrays fit");
     const int nb = (int)((n + SB - 1) / SB);
-    const bool fwd = kind_fwd(kind);
-    const int tid = threadIdx.x, lane = tid & 63, row = tid & 63, kq = tid >> 6;
-    const bool rowmajor = kind <= 1;
+    const bool fwd = kind_fwd(kind), rowmajor = kind <= 1, unit = kind == 1 || kind == 2;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const T zero = CX<T>::make(0, 0), one = CX<T>::make(1, 0);
     auto order = [&](int q) { return fwd ? q : nb - 1 - q; };
-    auto blen = [&](int b) { return (int)std::min((long)SB, n - (long)b * SB); };
-    // the operator entry (row i of block b, column k of block a), zero outside
-    auto entry = [&](int b, int a, int i, int k) -> T {
-        const long gi = (long)b * SB + i, gk = (long)a * SB + k;
-        return ldz(gi < n && gk < n, rowmajor ? LU + gi * lda + gk : LU + gk * lda + gi);
-    };
-    auto wait_flag = [&](int *f) {
-        long spins = 0;
-        while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-            __builtin_amdgcn_s_sleep(1);
-            if (++spins > (1L << 26)) { __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
+    auto spin = [&](long &spins) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > (1L << 24)) {
+            __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
         }
+        return true;
     };
-    auto stage_y = [&](int a) {              // ys = y_a from B, zero padded
-        const long k0 = (long)a * SB;
-        const int kn = blen(a);
-        for (int idx = tid; idx < SB * MAXRHS; idx += 256) {
-            const int i = idx / MAXRHS, r = idx % MAXRHS;
-            ys[i][r] = ldz(i < kn && r < nrhs, B + (k0 + i) * ldb + r);
-        }
-    };
-    if (blockIdx.x > 0) {
-        // ---- helper of the q-th block: the blocks published before its predecessor --------------------------
-        const int q = blockIdx.x - 1;
-        if (q < 2) return;
-        const int b = order(q), bn = blen(b);
-        T s[MAXRHS];
-#pragma unroll
-        for (int r = 0; r < MAXRHS; ++r) s[r] = zero;
-        int done = 0;
-        const int nbulk = q - 1;
-        while (done < nbulk) {
-            if (tid == 0) {
-                wait_flag(yflag + order(done));
-                int pp = done + 1;
-                while (pp < nbulk && __hip_atomic_load(yflag + order(pp), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0) ++pp;
-                s_avail = pp;
+    auto opp = [&](long gi, long gk) { return rowmajor ? LU + gi * lda + gk : LU + gk * lda + gi; };
+    const long kstep = rowmajor ? 1 : lda;   // address step along a row of the operator
+
+    if (blockIdx.x > (unsigned)nb) {
+        // ---- prefetcher: workgroups past the helpers whose index is a multiple of 8 run on the chain's XCD
+        // (workgroups are dealt to the 8 XCDs in turn); they read the tiles the chain's loaders will ask for a few
+        // blocks ahead, so that those loads hit that XCD's L2 instead of waiting on HBM
+        const int first = (nb + 1 + 7) & ~7;
+        if ((blockIdx.x & 7) || (int)blockIdx.x < first) return;
+        const int j = (blockIdx.x - first) >> 3;
+        const int stride = (gridDim.x - first + 7) >> 3;
+        float sum = 0.f;
+        for (int q = j; q < nb; q += stride) {
+            // the chain loads block q's tiles during step q - 1: start once y of block q - TRSV_LEAD is out
+            if (q >= TRSV_LEAD && lane == 0 && wv == 0) {
+                const T *w0 = dst + (long)order(q - TRSV_LEAD) * SB * nrhs;
+                T v;
+                long spins = 0;
+                int nap = 1;
+                while (!take(w0, v)) {
+                    for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(2);
+                    nap = nap < 16 ? 2 * nap : 16;
+                    if (++spins > (1L << 22)) break;
+                }
             }
             __syncthreads();
-            const int avail = s_avail;
-            for (int pp = done; pp < avail; ++pp) {
-                const int a = order(pp);
-                T m[16];
+            const int b = order(q);
+            const long r0 = (long)b * SB;
+            // the diagonal block, its coupling to block q + 1, and Op_{q+1, q-1}: rows (8 wv + u) of a row-major
+            // operator, columns of a column-major one, lane along the contiguous index
+            auto touch = [&](long rr0, long cc0) __attribute__((always_inline)) {
+                const T *base = opp(rr0, cc0);
+                const int lim = (int)std::min((long)SB, n - (rowmajor ? rr0 : cc0)) - 1;     // last slow index
+                const int f = (int)std::min((long)lane, n - 1 - (rowmajor ? cc0 : rr0));
 #pragma unroll
-                for (int u = 0; u < 16; ++u) m[u] = entry(b, a, row, kq * 16 + u);
-                stage_y(a);
-                __syncthreads();
-#pragma unroll
-                for (int u = 0; u < 16; ++u)
-#pragma unroll
-                    for (int r = 0; r < MAXRHS; ++r) s[r] = tadd(s[r], tmul(m[u], ys[kq * 16 + u][r]));
-                __syncthreads();
+                for (int u = 0; u < TE; ++u) sum += base[(long)std::min(wv * TE + u, lim) * lda + f].x;
+            };
+            touch(r0, r0);
+            if (q + 1 < nb) {
+                const long r1 = (long)order(q + 1) * SB;
+                touch(r1, r0);
+                if (q >= 1) touch(r1, (long)order(q - 1) * SB);
             }
-            done = avail;
+        }
+        if (sum == -1.2345e-30f) __hip_atomic_store(abort + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    if (blockIdx.x > 0) {
+        // ---- helper of the q-th block -------------------------------------------------------------------------
+        const int q = blockIdx.x - 1;
+        if (q <= HD) return;
+        T (*red)[NR][SB] = reinterpret_cast<T (*)[NR][SB]>(&Ms[0][0]);
+        const int b = order(q), nbulk = q - HD;
+        // thread (row lane, columns 8 wv + u of each block a); a row past the matrix reads the last row (its sums
+        // are never published)
+        auto load_htile = [&](T (&m)[TE], int a) __attribute__((always_inline)) {
+            const long gi = std::min((long)b * SB + lane, n - 1), k0 = (long)a * SB + wv * TE;
+            const T *p = opp(gi, k0);
+            if (k0 + TE <= n) {
+#pragma unroll
+                for (int u = 0; u < TE; ++u) m[u] = p[u * kstep];
+            } else {                                 // columns past the matrix meet zero y
+                const int ul = (int)(n - 1 - k0);
+#pragma unroll
+                for (int u = 0; u < TE; ++u) m[u] = p[(u < ul ? u : ul) * kstep];
+            }
+        };
+        T s[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) s[r] = zero;
+        // lane l < 8 NR of wave wv fetches y (row 8 wv + l / NR of block a, right-hand side l % NR; zero for the
+        // right-hand sides past nrhs)
+        const int lu_ = lane / NR, lr = lane % NR;
+        T *yb = &Cs[0][0] + wv * 64;         // this wave's 64 values in LDS
+        auto step = [&](int pp, const T (&mc)[TE], T (&mn)[TE]) __attribute__((always_inline)) {
+            const int a = order(pp);
+            const long gk = (long)a * SB + wv * TE + lu_;
+            const bool need = lane < TE * NR && lr < nrhs && gk < n;
+            // one lane watches the first value of the wave's slice, backing off while the chain is far away (a
+            // lighter load on the memory system than every lane polling); then every lane reads its own
+            if (lane == 0 && (long)a * SB + wv * TE < n) {         // (a slice past the matrix needs nothing)
+                const T *w0 = dst + ((long)a * SB + wv * TE) * nrhs;
+                T v;
+                long spins = 0;
+                int nap = 1;
+                while (!take(w0, v)) {
+                    for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(2);
+                    nap = nap < 16 ? 2 * nap : 16;
+                    if (++spins > (1L << 22)) {
+                        __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+            }
+            T yv = zero;
+            bool ok = !need || take(dst + gk * nrhs + lr, yv);
+            if (pp + 1 < nbulk) load_htile(mn, order(pp + 1));      // in flight behind y_a
+            long spins = 0;
+            while (!wave_all(ok)) {
+                if (!spin(spins)) break;
+                if (!ok) ok = take(dst + gk * nrhs + lr, yv);
+            }
+            if (!need) yv = zero;
+            yb[lane] = yv;
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int u = 0; u < TE; ++u)
+#pragma unroll
+                for (int r = 0; r < NR; ++r) s[r] = tadd(s[r], tmul(mc[u], yb[u * NR + r]));
+            __builtin_amdgcn_wave_barrier();
+        };
+        T m0[TE], m1[TE];
+        load_htile(m0, order(0));
+        for (int pp = 0; pp < nbulk; pp += 2) {
+            step(pp, m0, m1);
+            if (pp + 1 < nbulk) step(pp + 1, m1, m0);
         }
 #pragma unroll
-        for (int r = 0; r < MAXRHS; ++r) part[kq][row][r] = s[r];
+        for (int r = 0; r < NR; ++r) red[wv][r][lane] = s[r];
         __syncthreads();
-        T *Pb = P + (size_t)b * SB * MAXRHS;
-        for (int idx = tid; idx < bn * MAXRHS; idx += 256) {
-            const int i = idx / MAXRHS, r = idx % MAXRHS;
-            Pb[idx] = tadd(tadd(part[0][i][r], part[1][i][r]), tadd(part[2][i][r], part[3][i][r]));
-        }
-        __syncthreads();
-        if (tid == 0) {
-            __threadfence();
-            __hip_atomic_store(pflag + b, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (wv < nrhs && (long)b * SB + lane < n) {
+            T v = zero;
+#pragma unroll
+            for (int sl = 0; sl < NWV; ++sl) v = tadd(v, red[sl][wv][lane]);
+            put(P + ((size_t)b * SB + lane) * nrhs + wv, v);
         }
         return;
     }
+
     // ---- the chain --------------------------------------------------------------------------------------------
-    // The next block's diagonal operator and coupling, loaded a block ahead with coalesced reads: thread t holds
-    // entries (rows 16 (t / 64) + u, column t % 64) of a row-major operator, (row t % 64, columns 16 (t / 64) + u) of
-    // a column-major one.
-    T md[16], mo[16];
-    auto ij = [&](int u, int &i, int &k) __attribute__((always_inline)) {
-        if (rowmajor) { i = kq * 16 + u; k = lane; }
-        else { i = lane; k = kq * 16 + u; }
-    };
-    auto fetch_block = [&](T (&m)[16], int b, int a) __attribute__((always_inline)) {
-        if ((long)(b + 1) * SB <= n && (long)(a + 1) * SB <= n) {          // a full block: no guards
-            const long i0 = (long)b * SB, k0 = (long)a * SB;
-            const T *p = rowmajor ? LU + (i0 + kq * 16) * lda + k0 + lane : LU + (k0 + kq * 16) * lda + i0 + lane;
+    if (wv >= 4) {
+        // ---- loaders: thread (rows 16 lw + u, column lane) of a row-major operator, (row lane, columns 16 lw + u)
+        // of a column-major one, so that every load instruction reads whole lines. The loads are unconditional
+        // (indices clamped into the matrix) so that they all issue at once; entries outside the matrix or the
+        // diagonal block's strict triangle are zeroed when staged.
+        const int lw = __builtin_amdgcn_readfirstlane(wv - 4);
+        auto ij = [&](int u, int &i, int &k) __attribute__((always_inline)) {
+            if (rowmajor) { i = lw * TL + u; k = lane; }
+            else { i = lane; k = lw * TL + u; }
+        };
+        T md[TL], mc[TL], mx[TL];
+        // tile (rows of block rb, columns of block cb): element (slow index 16 lw + u, fast index lane) at
+        // base + slow * lda + fast. Past the matrix the indices are clamped: those entries meet only zero y or are
+        // zeroed when staged.
+        const int ld32 = (int)lda;           // offsets inside a tile's 64 x lda span fit 32 bits
+        auto load = [&](T (&m)[TL], int rb, int cb) __attribute__((always_inline)) {
+            const long r0 = (long)rb * SB, c0 = (long)cb * SB;
+            const T *base = opp(r0, c0) + (long)lw * TL * lda;    // uniform across the wave
+            if (r0 + SB <= n && c0 + SB <= n) {
 #pragma unroll
-            for (int u = 0; u < 16; ++u) m[u] = p[u * lda];
-        } else {
+                for (int u = 0; u < TL; ++u) m[u] = base[u * ld32 + lane];
+            } else {
+                const int rl = (int)(n - 1 - r0), cl = (int)(n - 1 - c0);
+                const int sl = (rowmajor ? rl : cl) - lw * TL, f = std::min(lane, rowmajor ? cl : rl);
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                int i, k;
-                ij(u, i, k);
-                m[u] = entry(b, a, i, k);
+                for (int u = 0; u < TL; ++u) m[u] = base[std::min(u, sl) * ld32 + f];
             }
-        }
-    };
-    auto fetch = [&](int q) __attribute__((always_inline)) {
-        const int b = order(q);
-        fetch_block(md, b, b);
-        if (q > 0) fetch_block(mo, b, order(q - 1));
-    };
-    const int w = tid >> 6;
-    const bool has0 = w < nrhs, has1 = w + 4 < nrhs;
-    // the right-hand side of block q less the helpers' partial sum, and the row's reciprocal diagonal, for this
-    // thread's row (lane) and right-hand sides w, w + 4: loads issued a block ahead, consumed by its solve
-    T bx0 = zero, bx1 = zero, px0 = zero, px1 = zero, rdn = one;
-    auto prefetch_rhs = [&](int q) __attribute__((always_inline)) {
-        const int b = order(q), bn = blen(b);
-        const long i0 = (long)b * SB;
-        const bool in = lane < bn;
-        if (has0) bx0 = ldz(in, B + (i0 + lane) * ldb + w);
-        if (has1) bx1 = ldz(in, B + (i0 + lane) * ldb + w + 4);
-        if (rdg) rdn = ldz(i0 + lane < n, rdg + i0 + lane);
-        px0 = px1 = zero;
-        if (q >= 2) {
-            if (lane == 0) wait_flag(pflag + b);
-            const T *Pb = P + (size_t)b * SB * MAXRHS + lane * MAXRHS;
-            if (has0) px0 = ldz(in, Pb + w);
-            if (has1) px1 = ldz(in, Pb + w + 4);
-        }
-    };
-    fetch(0);
-    prefetch_rhs(0);
-    for (int q = 0; q < nb; ++q) {
-        const int b = order(q), bn = blen(b);
-        const long i0 = (long)b * SB;
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
+        };
+        // entry u of the diagonal block b's operator kept: inside the matrix and in the strict triangle
+        auto keep = [&](int u, int b) __attribute__((always_inline)) {
             int i, k;
             ij(u, i, k);
-            Ms[i][k] = md[u];
-            if (q >= 1) Mo[i][k] = mo[u];
-        }
-        if (q + 1 < nb) fetch(q + 1);                    // in flight during this block's substitution
-        __syncthreads();
-        auto solve = [&](int r, T v, T pv) __attribute__((always_inline)) {
-            v = tsub(v, pv);
-            if (q >= 1) {                                // the predecessor's y is in ys: sixteen loads in flight
-                T sa[4] = {zero, zero, zero, zero};
-                for (int k0 = 0; k0 < SB; k0 += 16) {
-                    T mv[16], yv[16];
-#pragma unroll
-                    for (int j = 0; j < 16; ++j) { mv[j] = Mo[lane][k0 + j]; yv[j] = ys[k0 + j][r]; }
-#pragma unroll
-                    for (int j = 0; j < 16; ++j) sa[j & 3] = tadd(sa[j & 3], tmul(mv[j], yv[j]));
-                }
-                v = tsub(v, tadd(tadd(sa[0], sa[1]), tadd(sa[2], sa[3])));
-            }
-            v = tmul(v, rdn);
-            return fwd ? subst<true>(v, Ms, bn, lane, rdn) : subst<false>(v, Ms, bn, lane, rdn);
+            const int m = (int)std::min((long)SB, n - (long)b * SB);
+            return i < m && k < m && (fwd ? k < i : k > i);
         };
-        // wave w takes right-hand sides w and w + 4, lane i row i
-        T t0 = zero, t1 = zero;
-        if (has0) t0 = solve(w, bx0, px0);
-        if (has1) t1 = solve(w + 4, bx1, px1);
-        __syncthreads();                                 // every read of ys, Mo and Ms done
-        if (has0) ys[lane][w] = t0;
-        if (has1) ys[lane][w + 4] = t1;
-        if (lane < bn) {
-            if (has0) B[(i0 + lane) * ldb + w] = t0;
-            if (has1) B[(i0 + lane) * ldb + w + 4] = t1;
+        load(md, order(0), order(0));
+        if (nb > 1) load(mc, order(1), order(0));
+        for (int q = 0; q < nb; ++q) {
+            const int b = order(q), b1 = q + 1 < nb ? order(q + 1) : -1, b2 = q + 2 < nb ? order(q + 2) : -1;
+            __syncthreads();                             // A: the solvers are done with the last tiles
+#pragma unroll
+            for (int u = 0; u < TL; ++u) {
+                int i, k;
+                ij(u, i, k);
+                Ms[i][k] = csel(keep(u, b), tmul(md[u], rds[i]), zero);
+                Cs[i][k] = b1 >= 0 ? mc[u] : zero;
+            }
+            __syncthreads();                             // B: the tiles are in LDS
+            if (b1 >= 0) {
+                load(md, b1, b1);
+                if (b2 >= 0) load(mc, b2, b1);
+            }
+            if (b2 >= 0) load(mx, b2, b);                // Op_{q+2, q}
+            __syncthreads();                             // C: y_q is in LDS; Ms and Cs are free until A
+            T part[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) part[r] = zero;
+            if (b2 >= 0) {
+                if (!rowmajor) {                         // lane's row, columns 16 lw + u
+#pragma unroll
+                    for (int u = 0; u < TL; ++u)
+#pragma unroll
+                        for (int r = 0; r < NR; ++r) part[r] = tadd(part[r], tmul(mx[u], ys[r][lw * TL + u]));
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) lp[lw][r][lane] = part[r];
+                } else {
+                    // the wave's sixteen rows through its own rows of Ms: lane (row 16 lw + lane / 4, columns
+                    // 16 (lane % 4) + j)
+#pragma unroll
+                    for (int u = 0; u < TL; ++u) Ms[lw * TL + u][lane] = mx[u];
+                    __builtin_amdgcn_wave_barrier();
+                    const int i = lw * TL + (lane >> 2), c = lane & 3;
+#pragma unroll
+                    for (int j = 0; j < TL; ++j) {
+                        const T x = Ms[i][c * TL + j];
+#pragma unroll
+                        for (int r = 0; r < NR; ++r) part[r] = tadd(part[r], tmul(x, ys[r][c * TL + j]));
+                    }
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) lp[c][r][i] = part[r];
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < NR; ++r) lp[lw][r][lane] = zero;
+            }
         }
-        __syncthreads();
-        // wave 3 raises the flag (its release waits on its own outstanding memory operations only) before it
-        // issues its prefetches; the other waves prefetch at once
-        if (tid == 192) __hip_atomic_store(yflag + b, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        if (q + 1 < nb) prefetch_rhs(q + 1);
+        return;
+    }
+    // ---- solvers ------------------------------------------------------------------------------------------------
+    T bx[NW], px[NW], acc[NW];                   // lane's row: source, P, - the chain's products (block q)
+    T rdl = one;                                 // 1 / U_ii of the lane's row of the next block
+    bool act[NW];
+    int rr[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        rr[w] = wv + 4 * w;
+        act[w] = rr[w] < nrhs;
+        bx[w] = px[w] = acc[w] = zero;
+    }
+    auto issue_src = [&](int q) __attribute__((always_inline)) {      // source values and 1 / U_ii of block q
+        const long gi = std::min((long)order(q) * SB + lane, n - 1);    // rows past n: zeroed where used
+#pragma unroll
+        for (int w = 0; w < NW; ++w) bx[w] = src[gi * lds + std::min(rr[w], nrhs - 1)];
+        const T rv = rdg[gi];                   // read either way: a select of values, not of addresses
+        rdl = csel(unit, CX<T>::make(1, 0), rv);
+    };
+    auto has_p = [&](int q) { return q > HD; };                       // P of block q comes from a helper
+    issue_src(0);
+    if (wv == 0) rds[lane] = (long)order(0) * SB + lane < n ? rdl : zero;
+    for (int q = 0; q < nb; ++q) {
+        const long i0 = (long)order(q) * SB;
+        const bool inrow = i0 + lane < n;
+        T t[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) t[w] = csel(inrow && act[w], tmul(tadd(tsub(bx[w], px[w]), acc[w]), rdl), zero);
+        __syncthreads();                                 // A
+        // block q + 1: its source, 1 / U_ii, P (asked for now, checked after C), the loaders' partial sums
+        const long g1 = (long)order(q + 1 < nb ? q + 1 : q) * SB + lane;
+        const bool p1 = q + 1 < nb && has_p(q + 1) && g1 < n;
+        if (q + 1 < nb) issue_src(q + 1);
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            px[w] = p1 && act[w] ? fetch(P + g1 * nrhs + rr[w]) : zero;
+            T l = zero;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) l = tadd(l, lp[j][w * 4 + (wv & 3)][lane]);
+            acc[w] = q >= 1 ? tsub(zero, l) : zero;      // Op_{q+1, q-1} y_{q-1}, from the loaders
+        }
+        __syncthreads();                                 // B: block q's tiles in LDS
+        if (fwd) subst_fused<true, NW>(t, acc, Ms, Cs, lane);
+        else subst_fused<false, NW>(t, acc, Ms, Cs, lane);
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            ys[w * 4 + (wv & 3)][lane] = t[w];           // zero for the right-hand sides past nrhs
+            if (act[w] && inrow) put(dst + (i0 + lane) * nrhs + rr[w], t[w]);
+        }
+        __syncthreads();                                 // C: y_q in LDS
+        if (q + 1 < nb) {
+            bool ok = true;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) ok = ok && (!p1 || !act[w] || is_set(px[w]));
+            long spins = 0;
+            while (!wave_all(ok)) {
+                if (!spin(spins)) break;
+                ok = true;
+#pragma unroll
+                for (int w = 0; w < NW; ++w)
+                    if (p1 && act[w] && !is_set(px[w])) ok = take(P + g1 * nrhs + rr[w], px[w]) && ok;
+            }
+            if (wv == 0) rds[lane] = (long)order(q + 1) * SB + lane < n ? rdl : zero;   // staged after the next A
+        }
     }
 }
 
@@ -717,16 +913,24 @@ __global__ void k_rdiag(const T *LU, long lda, long n, T *rd)
     if (i < n) rd[i] = trecip(LU[i * lda + i]);
 }
 
-// out[perm[i]] = in[i] (scatter, trans) or out[i] = in[perm[i]] (gather)
+// out[perm[i]] = in[i] (scatter, trans) or out[i] = in[perm[i]] (gather); perm NULL: a copy. Leading dimensions
+// ldi (in) and ldo (out).
 template <typename T>
-__global__ void k_permute(const T *in, T *out, long n, int nrhs, long ld, const int *perm, int scatter)
+__global__ void k_permute(const T *in, long ldi, T *out, long ldo, long n, int nrhs, const int *perm, int scatter)
 {
     const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (idx >= n * nrhs) return;
     const long i = idx / nrhs;
     const int r = (int)(idx % nrhs);
-    if (scatter) out[(long)perm[i] * ld + r] = in[i * ld + r];
-    else out[i * ld + r] = in[(long)perm[i] * ld + r];
+    const long p = perm ? (long)perm[i] : i;
+    if (scatter) out[p * ldo + r] = in[i * ldi + r];
+    else out[i * ldo + r] = in[p * ldi + r];
+}
+
+__global__ void k_fill_unset(unsigned long long *p, long n)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = ~0ull;
 }
 
 inline unsigned blocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -846,11 +1050,12 @@ int getrf_tp(T *A, long n, long lda, int *ipiv, int *perm, int *info, void *work
     return M25X_OK;
 }
 
-template <typename T> size_t getrs_work(long n, long ldb)
+// Solve workspace: y of the two solves (W1, W2) and their partial sums (P1, P2), written as self-marking values over
+// UNSET words; the gathered right-hand sides (W0); 1 / U_ii; the permutation and the abort word.
+template <typename T> size_t getrs_work(long n, long)
 {
     const long nb = (n + SB - 1) / SB;
-    return (size_t)n * ldb * sizeof(T) + (size_t)n * sizeof(T) + (size_t)nb * SB * MAXRHS * sizeof(T) +
-           sizeof(int) * ((size_t)n + 4 * nb + 64);
+    return sizeof(T) * (3 * (size_t)n * MAXRHS + 2 * (size_t)nb * SB * MAXRHS + (size_t)n) + sizeof(int) * ((size_t)n + 64);
 }
 
 // ---- launch timing of the triangular solves (benchmarks; one host thread) -----------------------------------
@@ -877,12 +1082,18 @@ int tt_mark(hipStream_t s)
 }
 
 template <typename T>
-int trsv_launch(hipStream_t s, int nb, const T *LU, long lda, long n, const T *rd, T *B, long ldb, int nrhs, int kind,
-                T *P, int *yflag, int *pflag, int *abort)
+int trsv_launch(hipStream_t s, int nb, const T *LU, long lda, long n, const T *rd, const T *src, long lds, T *dst,
+                int nrhs, int kind, T *P, int *abort)
 {
     if (int rc = tt_mark(s)) return rc;
-    hipLaunchKernelGGL(k_trsv<T>, dim3(nb + 1), dim3(256), 0, s, LU, lda, n, rd, B, ldb, nrhs, kind, P, yflag, pflag,
-                       abort);
+    // the chain, nb helpers, then up to the next multiple of 8 and TRSV_NPF groups of 8 (one prefetcher each)
+    const unsigned grid = (unsigned)(((nb + 1 + 7) & ~7) + 8 * TRSV_NPF);
+    if (nrhs <= 4)
+        hipLaunchKernelGGL((k_trsv<T, 4>), dim3(grid), dim3(TB), 0, s, LU, lda, n, rd, src, lds, dst, nrhs, kind, P,
+                           abort);
+    else
+        hipLaunchKernelGGL((k_trsv<T, TrsvMaxRhs<T>::NR>), dim3(grid), dim3(TB), 0, s, LU, lda, n, rd, src, lds, dst,
+                           nrhs, kind, P, abort);
     if (int rc = check_launch("getrs/trsv")) return rc;
     if (g_tt.on) {
         const double tri = kind_fwd(kind) ? 0.5 * n * (n - 1) : 0.5 * n * (n + 1);
@@ -896,40 +1107,42 @@ int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, 
             void *work, size_t wbytes, hipStream_t s)
 {
     if (nrhs < 1 || nrhs > MAXRHS) return fail(M25X_EINVAL, "getrs: 1 to %d right-hand sides, not %d", MAXRHS, nrhs);
+    if (nrhs > TrsvMaxRhs<T>::NR) {                      // in two halves (the solve's LDS holds fewer)
+        const int h = TrsvMaxRhs<T>::NR;
+        if (int rc = getrs_t<T>(LU, n, lda, ipiv, perm_in, B, ldb, h, trans, work, wbytes, s)) return rc;
+        return getrs_t<T>(LU, n, lda, ipiv, perm_in, B + h, ldb, nrhs - h, trans, work, wbytes, s);
+    }
     if (ldb < nrhs) return fail(M25X_EINVAL, "getrs: ldb < nrhs");
     if (!work || wbytes < getrs_work<T>(n, ldb))
         return fail(M25X_EINVAL, "getrs: workspace of %zu bytes, %zu needed", wbytes, getrs_work<T>(n, ldb));
     const int nb = (int)((n + SB - 1) / SB);
-    const size_t tmp_bytes = (size_t)n * ldb * sizeof(T);
-    T *tmp = (T *)work;
-    T *rd = tmp + (size_t)n * ldb;
-    T *P = rd + n;
-    int *perm = (int *)(P + (size_t)nb * SB * MAXRHS);
-    int *flags = perm + n;                      // y flags and partial-sum flags of the two solves, abort
-    int *abort = flags + 4 * nb;
-    hipLaunchKernelGGL(k_zero_ints, dim3(blocks(4 * nb + 1, 256)), dim3(256), 0, s, flags, 4 * nb + 1);
+    const size_t wn = (size_t)n * MAXRHS, pn = (size_t)nb * SB * MAXRHS;
+    T *W1 = (T *)work, *W2 = W1 + wn, *P1 = W2 + wn, *P2 = P1 + pn, *W0 = P2 + pn, *rd = W0 + wn;
+    int *perm = (int *)(rd + n);
+    int *abort = perm + n;
+    const long words = (long)((2 * wn + 2 * pn) * sizeof(T) / 8);
+    hipLaunchKernelGGL(k_fill_unset, dim3(blocks(words, 256)), dim3(256), 0, s, (unsigned long long *)W1, words);
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, abort, 1);
     hipLaunchKernelGGL(k_rdiag<T>, dim3(blocks(n, 256)), dim3(256), 0, s, LU, lda, n, rd);
     if (!perm_in) {
         hipLaunchKernelGGL(k_perm, dim3(1), dim3(256), 0, s, ipiv, n, perm);
         perm_in = perm;
     }
     if (int rc = check_launch("getrs")) return rc;
-    int *f0 = flags, *f1 = flags + 2 * nb;
+    const unsigned pg = blocks(n * nrhs, 256);
     int rc;
     if (trans) {
-        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, rd, B, ldb, nrhs, 0, P, f0, f0 + nb, abort))) return rc;
-        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, nullptr, B, ldb, nrhs, 1, P, f1, f1 + nb, abort))) return rc;
-        hipLaunchKernelGGL(k_permute<T>, dim3(blocks(n * nrhs, 256)), dim3(256), 0, s, (const T *)B, tmp, n, nrhs, ldb,
-                           perm_in, 1);
+        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, rd, B, ldb, W1, nrhs, 0, P1, abort))) return rc;
+        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, rd, W1, nrhs, W2, nrhs, 1, P2, abort))) return rc;
+        hipLaunchKernelGGL(k_permute<T>, dim3(pg), dim3(256), 0, s, (const T *)W2, (long)nrhs, B, ldb, n, nrhs, perm_in, 1);
     } else {
-        hipLaunchKernelGGL(k_permute<T>, dim3(blocks(n * nrhs, 256)), dim3(256), 0, s, (const T *)B, tmp, n, nrhs, ldb,
-                           perm_in, 0);
-        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, nullptr, tmp, ldb, nrhs, 2, P, f0, f0 + nb, abort))) return rc;
-        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, rd, tmp, ldb, nrhs, 3, P, f1, f1 + nb, abort))) return rc;
+        hipLaunchKernelGGL(k_permute<T>, dim3(pg), dim3(256), 0, s, (const T *)B, ldb, W0, (long)nrhs, n, nrhs, perm_in, 0);
+        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, rd, W0, nrhs, W1, nrhs, 2, P1, abort))) return rc;
+        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, rd, W1, nrhs, W2, nrhs, 3, P2, abort))) return rc;
+        hipLaunchKernelGGL(k_permute<T>, dim3(pg), dim3(256), 0, s, (const T *)W2, (long)nrhs, B, ldb, n, nrhs,
+                           (const int *)nullptr, 0);
     }
-    if (int rc = check_launch("getrs")) return rc;
-    M25X_HIP(hipMemcpyAsync(B, tmp, tmp_bytes, hipMemcpyDeviceToDevice, s));
-    return M25X_OK;
+    return check_launch("getrs");
 }
 
 }  // namespace

@@ -310,7 +310,7 @@ def getrs(F, B, trans=1):
     dt = L.C128 if F.dtype == c128 else L.C64
     for c0 in range(0, k, 8):
         kc = min(8, k - c0)
-        X = B[:, c0:c0 + kc].to(F.dtype).contiguous()
+        X = B[:, c0:c0 + kc].to(F.dtype, copy=True).contiguous()      # solved in place: never the caller's B
         wb = L.lib().m25x_getrs_work_bytes(dt, n, kc)
         work = torch.empty(wb, dtype=torch.uint8, device=B.device)
         L.call('m25x_getrs', dt, ptr(F.LU), n, F.LU.stride(0), ptr(F.ipiv), ptr(F.perm), ptr(X), kc, kc,

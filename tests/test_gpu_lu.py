@@ -148,3 +148,26 @@ def test_getrf_c64_tournament(gpu, n):
     F = _factor(gpu, Z, 'tournament')
     assert int(F.info.item()) == 0
     _check_tournament(gpu, Z, F, 256, 4, 5e-5, 1e-4, False)
+
+
+@pytest.mark.parametrize('dt', [np.complex64, np.complex128])
+def test_getrs_right_hand_side_counts(gpu, dt):
+    """The solves for 1 to 8 right-hand sides at sizes with and without a partial last 64-row block and with long
+    block chains (the chain workgroup, its helpers and its look-ahead couplings all in play), both transposes,
+    against NumPy's solve; the caller's B is left as it was."""
+    dv = gpu
+    tol = 1e-4 if dt == np.complex64 else 1e-10
+    for n in (1, 63, 64, 200, 777, 3001):
+        Z = seeded_matrix(n, dtype=dt)
+        F = _factor(dv, Z, 'tournament')
+        Zc = Z.astype(complex)
+        rng = np.random.default_rng(n)
+        for k in (1, 3, 4, 5, 8):
+            R = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
+            Rd = torch.from_numpy(R).to(dv.device())
+            R0 = Rd.clone()
+            for trans in (1, 0):
+                X = dv.getrs(F, Rd, trans=trans).cpu().numpy()
+                assert torch.equal(Rd, R0), (n, k, trans)
+                x = np.linalg.solve(Zc if trans else Zc.T, R)
+                assert np.abs(X - x).max() <= tol * np.abs(x).max(), (n, k, trans)
