@@ -179,6 +179,17 @@ long m25x_trsv_times(float *ms, double *bytes, long cap);
  * The FP64 products of the GMRES refinement (src/mom25d/fast.py:1004-1007, A = Z @ V). */
 int m25x_zgemm_nk(const double *Z, long n, long ld, const double *X, int k, double *Y, void *stream);
 
+/* Workspace bytes of m25x_cgs2. */
+size_t m25x_cgs2_work_bytes(void);
+
+/* One Arnoldi step of the batched GMRES, per column c of the k <= 8 systems: classical Gram-Schmidt applied twice
+ * to w (n x k complex128 row-major, overwritten) against the nv basis vectors V (nv x n x k, vector i at
+ * V + 2 i n k doubles): hcol[i][c] = h1 + h2 for i < nv, hcol[nv][c] = |w| (real), vnext = w / |w|. Sums run over
+ * fixed row blocks and then in block order (deterministic). Replaces the orthogonalisation of the reference's
+ * gmres_batched (src/mom25d/fast.py:1437-1500, modified Gram-Schmidt: the same basis to rounding). */
+int m25x_cgs2(const double *V, long n, int k, int nv, double *w, double *hcol, double *vnext, void *work,
+              size_t work_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

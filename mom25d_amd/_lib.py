@@ -53,6 +53,8 @@ _SIGS = {
     'm25x_trsv_timing': ([I], I),
     'm25x_trsv_times': ([P, P, L], L),
     'm25x_zgemm_nk': ([P, L, L, P, I, P, P], I),
+    'm25x_cgs2_work_bytes': ([], S),
+    'm25x_cgs2': ([P, L, I, I, P, P, P, P, S, P], I),
 }
 
 _lib = None

@@ -351,3 +351,21 @@ def zgemm_nk(Z, X):
         L.call('m25x_zgemm_nk', ptr(Z), n, Z.stride(0), ptr(Xc), kc, ptr(Yc), stream())
         Y[:, c0:c0 + kc] = Yc
     return Y[:, 0] if one else Y
+
+
+_CGS2_WORK = {}
+
+
+def cgs2(V, nv, W, hcol):
+    """One Arnoldi step on the device: W (n x k complex128, overwritten) orthogonalised against V[:nv] by classical
+    Gram-Schmidt applied twice; hcol[:nv + 1] (nv + 1, k) receives the coefficients and |W| (real); V[nv] = W / |W|.
+    V: (m, n, k) complex128 contiguous, k <= 8. Calls on one device share a workspace: one stream at a time."""
+    m, n, k = V.shape
+    if not (V.is_contiguous() and W.is_contiguous() and hcol.is_contiguous()) or W.shape != (n, k) or nv >= m:
+        raise ValueError("cgs2: V (m, n, k), W (n, k) contiguous, nv < m")
+    dev = V.device
+    work = _CGS2_WORK.get(dev)
+    if work is None:
+        work = _CGS2_WORK[dev] = torch.empty(L.lib().m25x_cgs2_work_bytes(), dtype=torch.uint8, device=dev)
+    L.call('m25x_cgs2', ptr(V), n, k, nv, ptr(W), ptr(hcol), ptr(V[nv]), ptr(work), work.numel(), stream())
+

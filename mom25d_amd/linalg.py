@@ -3,12 +3,14 @@
 Right-preconditioned, k systems in lockstep: every operator / preconditioner call is one (N x k_active) product on
 the GPU (the FP64 Z product of m25x_zgemm_nk, the complex64 factor's solves of m25x_getrs). The Krylov basis stays
 on the device and is orthogonalised by classical Gram-Schmidt applied twice (the reference uses modified
-Gram-Schmidt: the same basis to rounding, in four batched products per iteration instead of 2 (j + 1)); the small
-Hessenberg columns come to the host once per iteration for the conjugated Givens rotations, the convergence test
-and the stall cut-off, as the reference computes them.
+Gram-Schmidt: the same basis to rounding), fused in one call (m25x_cgs2) for up to 8 systems; the small Hessenberg
+columns come to the host once per iteration for the conjugated Givens rotations, the convergence test and the
+stall cut-off, as the reference computes them.
 """
 import numpy as np
 import torch
+
+from . import device as dv
 
 
 def _cnorm(X):
@@ -46,20 +48,24 @@ def gmres_batched(A, B, M=None, x0=None, rtol=1e-6, restart=30, maxiter=10, stal
         g[0] = beta
         V[0] = R / torch.from_numpy(beta).to(dev)[None, :]
         live, last, stalled = np.ones(ka, bool), np.full(ka, restart - 1), False
+        hbuf = torch.empty((restart + 1, ka), dtype=torch.complex128, device=dev)
         for j in range(restart):
             Zs[j] = M(V[j])
-            Wv = A(Zs[j])
-            # classical Gram-Schmidt twice (as stable as modified Gram-Schmidt, in four batched products
-            # instead of 2 (j + 1) small ones)
-            Vj = V[:j + 1]
-            h1 = (Vj.conj() * Wv[None]).sum(1)
-            Wv = Wv - (Vj * h1[:, None, :]).sum(0)
-            h2 = (Vj.conj() * Wv[None]).sum(1)
-            Wv = Wv - (Vj * h2[:, None, :]).sum(0)
-            hn = torch.linalg.vector_norm(Wv, dim=0)
-            hcol = torch.cat([h1 + h2, hn[None].to(torch.complex128)], 0)
-            V[j + 1] = Wv / torch.where(hn > 0, hn, torch.ones_like(hn))
-            hc = hcol.cpu().numpy()
+            Wv = A(Zs[j]).contiguous()
+            # classical Gram-Schmidt twice (as stable as modified Gram-Schmidt): one fused device call for up to 8
+            # systems and 63 basis vectors, batched tensor products beyond
+            if ka <= 8 and j + 1 < 64:
+                dv.cgs2(V, j + 1, Wv, hbuf)
+                hc = hbuf[:j + 2].cpu().numpy()
+            else:
+                Vj = V[:j + 1]
+                h1 = (Vj.conj() * Wv[None]).sum(1)
+                Wv = Wv - (Vj * h1[:, None, :]).sum(0)
+                h2 = (Vj.conj() * Wv[None]).sum(1)
+                Wv = Wv - (Vj * h2[:, None, :]).sum(0)
+                hn = torch.linalg.vector_norm(Wv, dim=0)
+                V[j + 1] = Wv / torch.where(hn > 0, hn, torch.ones_like(hn))
+                hc = torch.cat([h1 + h2, hn[None].to(torch.complex128)], 0).cpu().numpy()
             H[:j + 2, j] = hc
             for i in range(j):                                       # previous Givens rotations
                 H[i, j], H[i + 1, j] = (cs[i] * H[i, j] + sn[i] * H[i + 1, j],
