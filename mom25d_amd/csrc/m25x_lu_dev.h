@@ -81,7 +81,7 @@ template <typename T> __device__ inline double cabs1(T a) { return fabs((double)
 // L21 = A21 U11^-1 and the trailing matrix the usual update, on the matrix cores.
 constexpr int TW = 64;
 template <typename T> struct Tour;
-template <> struct Tour<double2> { static constexpr int RL = 128, AR = 2; };   // rows a leaf, children a node
+template <> struct Tour<double2> { static constexpr int RL = 256, AR = 4; };   // rows a leaf, children a node
 template <> struct Tour<float2> { static constexpr int RL = 256, AR = 4; };
 
 template <typename T> struct TNode {

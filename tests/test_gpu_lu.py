@@ -131,12 +131,12 @@ def _check_tournament(dv, Z, F, RL, AR, tol_fac, tol_solve, full_ipiv):
 
 @pytest.mark.parametrize('n', [1, 7, 64, 65, 129, 300, 1000, 2000])
 def test_getrf_c128_tournament_matches_oracle(gpu, n):
-    """complex128 tournament pivoting (leaves of 128 rows, binary tree): the oracle's interchanges exactly, the
+    """complex128 tournament pivoting (leaves of 256 rows, 4 children a node): the oracle's interchanges exactly, the
     factor within 1e-11, solves within 1e-10."""
     Z = seeded_matrix(n)
     F = _factor(gpu, Z, 'tournament')
     assert int(F.info.item()) == 0
-    _check_tournament(gpu, Z, F, 128, 2, 1e-11, 1e-10, True)
+    _check_tournament(gpu, Z, F, 256, 4, 1e-11, 1e-10, True)
 
 
 @pytest.mark.parametrize('n', [300, 1000, 2000, 5000])
