@@ -1,4 +1,4 @@
-// m25x_lu.hip - dense complex LU with partial pivoting on gfx950, and the triangular solves with its factor.
+// m25x_lu.hip - dense complex LU on gfx950, and the triangular solves with its factor.
 //
 // A is column-major (the row-major Z of the reference seen as A = Z^T, the layout of scipy's lu_factor(Z.T)).
 // Blocked right-looking. Two pivoting strategies:
@@ -16,8 +16,8 @@
 // grid barrier (k_tslu), its row moves on every column (k_apply_moves), then L21 = A21 U11^-1 by substitution,
 // U12 = L11^-1 A12 by substitution and A22 -= L21 U12 on the matrix cores (complex64 with FP64 accumulation: one
 // rounding per update).
-// The solves (getrs) run one workgroup per 64-row block with ready flags between blocks (no launch per block), each
-// block's diagonal solved by substitution in one wave.
+// The solves (getrs): one launch per triangle; a chain workgroup solves the 64-row diagonal blocks in order while
+// helper workgroups accumulate the products with earlier blocks, values handed over as self-marking words.
 #include <mutex>
 #include "m25x_lu_dev.h"
 
