@@ -49,6 +49,7 @@ __device__ inline u64 mag_bits(double2 a) { return (u64)__double_as_longlong(fab
 
 // GEPP of rows held in registers, two threads a row: in wave w, lanes L and L + 32 hold row 32 w + L % 32, the
 // thread of half h = L / 32 columns 2 j + h in x[j] (j < TW / 2), so both halves have equal work at every step.
+// Multipliers: the column times the pivot's reciprocal conj(p) / |p|^2 (LAPACK's getf2 scales by 1 / pivot).
 // nr rows over columns [0, nc). Rows never move: a row's position in the elimination order is tracked instead
 // (pos), and ties in the pivot search go to the earliest position (LAPACK's rule). Step k: every wave's best row
 // publishes itself in its slot (LDS) with its (magnitude, key); after one barrier every thread picks the best of
@@ -87,22 +88,48 @@ __device__ __attribute__((always_inline)) int reg_gepp(RowH<T> &x, int nr, int n
         u64 vb = cand ? mag_bits(mine) : 0ull;
         if (cand && !(vb <= (sizeof(T) == 8 ? 0x7f800000ull : 0x7ff0000000000000ull))) vb = 0;   // NaN: as zero
         int key = cand ? ((pos << 16) | row) : 0x7fffffff;
-        wave_best(vb, key);
-        if ((key & 0xffff) == row && key != 0x7fffffff) {     // this wave's best row publishes itself
-            auto pub = [&](auto jc) __attribute__((always_inline)) {
-                constexpr int j = decltype(jc)::value;
-                sh.rows[b][w][2 * j + h] = x.get(j);
-            };
-            Unroll<jk, TH>::run(pub);
+        int gk;
+        if constexpr (sizeof(T) == 8) {
+            // complex64: the magnitude's 32 bits above the complemented key, so one maximum picks the largest
+            // magnitude and, among equals, the smallest key (a dead row: 0, below every live one)
+            u64 pk = cand ? (vb << 32) | (u64)(0xffffffffu - (unsigned)key) : 0ull;
+            pk = max(pk, dpp_u64<0xB1>(pk));
+            pk = max(pk, dpp_u64<0x4E>(pk));
+            pk = max(pk, dpp_u64<0x141>(pk));
+            pk = max(pk, dpp_u64<0x140>(pk));
+            pk = max(max(readlane_u64(pk, 0), readlane_u64(pk, 16)), max(readlane_u64(pk, 32), readlane_u64(pk, 48)));
+            key = pk ? (int)(0xffffffffu - (unsigned)pk) : 0x7fffffff;
+            if ((key & 0xffff) == row && key != 0x7fffffff) {     // this wave's best row publishes itself
+                auto pub = [&](auto jc) __attribute__((always_inline)) {
+                    constexpr int j = decltype(jc)::value;
+                    sh.rows[b][w][2 * j + h] = x.get(j);
+                };
+                Unroll<jk, TH>::run(pub);
+            }
+            if (lane == 0) sh.vb[b][w] = pk;
+            __syncthreads();
+            u64 g = sh.vb[b][0];
+#pragma unroll
+            for (int q = 1; q < NW; ++q) g = max(g, sh.vb[b][q]);
+            gk = (int)(0xffffffffu - (unsigned)g);
+        } else {
+            wave_best(vb, key);
+            if ((key & 0xffff) == row && key != 0x7fffffff) {     // this wave's best row publishes itself
+                auto pub = [&](auto jc) __attribute__((always_inline)) {
+                    constexpr int j = decltype(jc)::value;
+                    sh.rows[b][w][2 * j + h] = x.get(j);
+                };
+                Unroll<jk, TH>::run(pub);
+            }
+            if (lane == 0) { sh.vb[b][w] = vb; sh.key[b][w] = key; }
+            __syncthreads();
+            u64 gv = sh.vb[b][0];
+#pragma unroll
+            for (int q = 1; q < NW; ++q) gv = max(gv, sh.vb[b][q]);
+            gk = 0x7fffffff;
+#pragma unroll
+            for (int q = 0; q < NW; ++q) gk = sh.vb[b][q] == gv ? min(gk, sh.key[b][q]) : gk;
         }
-        if (lane == 0) { sh.vb[b][w] = vb; sh.key[b][w] = key; }
-        __syncthreads();
-        u64 gv = sh.vb[b][0];
-#pragma unroll
-        for (int q = 1; q < NW; ++q) gv = max(gv, sh.vb[b][q]);
-        int gk = 0x7fffffff;
-#pragma unroll
-        for (int q = 0; q < NW; ++q) gk = sh.vb[b][q] == gv ? min(gk, sh.key[b][q]) : gk;
         const int ps = gk & 0xffff, p = gk >> 16;
         const T *prow = sh.rows[b][ps >> 5];
         if (row == ps) {
@@ -113,8 +140,13 @@ __device__ __attribute__((always_inline)) int reg_gepp(RowH<T> &x, int nr, int n
         if (live) {
             const T a = csel(h == hk, mine, other);
             if (pos == k) pos = p;
+            // the multiplier by the pivot's reciprocal conj(pv) / |pv|^2, as LAPACK's getf2 scales by 1 / pivot (a
+            // zero pivot leaves the column as it is)
             const T pv = prow[k];
-            const T l = (pv.x == 0 && pv.y == 0) ? a : tdiv(a, pv);
+            typedef typename CX<T>::R R;
+            const R d = pv.x * pv.x + pv.y * pv.y;
+            const R inv = d != (R)0 ? (R)1 / d : (R)0;
+            const T l = d != (R)0 ? tmul(a, CX<T>::make(pv.x * inv, -pv.y * inv)) : a;
             if (h == hk) x.set(jk, l);
             else if (hk == 0) x.set(jk, tsub(x.get(jk), tmul(l, prow[2 * jk + 1])));   // column k + 1 of half 1
             auto upd = [&](auto jc) __attribute__((always_inline)) {

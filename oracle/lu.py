@@ -25,8 +25,9 @@ def gepp_order(M):
         if p != k:
             M[[k, p]] = M[[p, k]]
             rows[k], rows[p] = rows[p], rows[k]
-        if M[k, k] != 0:
-            M[k + 1:, k] /= M[k, k]
+        if M[k, k] != 0:                                   # times the reciprocal conj(p) / |p|^2, as the GPU's GEPP
+            p = M[k, k]
+            M[k + 1:, k] *= np.conj(p) * (1.0 / (p.real * p.real + p.imag * p.imag))
         M[k + 1:, k + 1:] -= np.outer(M[k + 1:, k], M[k, k + 1:])
     return rows[:min(r, c)]
 
@@ -79,7 +80,9 @@ def getrf_tournament(A, TW=64, RL=128, AR=2):
         # the panel's top block has its pivots on the diagonal now: unpivoted LU of it
         T = A[j0:j0 + jb, j0:j0 + jb]
         for k in range(jb):
-            T[k + 1:, k] /= T[k, k]
+            p = T[k, k]
+            if p != 0:
+                T[k + 1:, k] *= np.conj(p) * (1.0 / (p.real * p.real + p.imag * p.imag))
             T[k + 1:, k + 1:] -= np.outer(T[k + 1:, k], T[k, k + 1:])
         U11 = np.triu(T)
         L11 = np.tril(T, -1) + np.eye(jb)
