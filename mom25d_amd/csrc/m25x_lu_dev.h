@@ -36,6 +36,15 @@ template <typename T> __device__ inline T tmul(T a, T b)
 {
     return CX<T>::make(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+// t - m y in four fused multiply-adds
+__device__ inline float2 tfms(float2 t, float2 m, float2 y)
+{
+    return make_float2(fmaf(-m.x, y.x, fmaf(m.y, y.y, t.x)), fmaf(-m.y, y.x, fmaf(-m.x, y.y, t.y)));
+}
+__device__ inline double2 tfms(double2 t, double2 m, double2 y)
+{
+    return make_double2(fma(-m.x, y.x, fma(m.y, y.y, t.x)), fma(-m.y, y.x, fma(-m.x, y.y, t.y)));
+}
 // reciprocal by Smith's algorithm
 template <typename T> __device__ inline T trecip(T b)
 {

@@ -56,10 +56,21 @@ __device__ inline u64 mag_bits(double2 a) { return (u64)__double_as_longlong(fab
 // the slots, takes its multiplier and updates. order[k] = the k-th pivot row; a pivot row's threads keep its step in
 // step_of. On return a pivot row holds its row of L \ U, the other rows their multipliers and Schur complement.
 // One barrier a step: the slots are double-buffered.
-template <typename T> __device__ inline T shfl_xor32(T v)
+// lane L's value to lane L ^ 32: v_permlane32_swap exchanges the upper half of its destination with the lower half of
+// its source (a VALU operation, no LDS round trip); with both the same register, lane L < 32 finds lane L + 32's
+// value in the second result and lane L >= 32 lane L - 32's in the first
+__device__ inline unsigned swap32(unsigned v)
 {
-    return CX<T>::make(__shfl_xor(v.x, 32), __shfl_xor(v.y, 32));
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (threadIdx.x & 32) ? r[0] : r[1];
 }
+__device__ inline float swap32(float v) { return __uint_as_float(swap32(__float_as_uint(v))); }
+__device__ inline double swap32(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return __longlong_as_double((long long)(((unsigned long long)swap32((unsigned)(b >> 32)) << 32) | swap32((unsigned)b)));
+}
+template <typename T> __device__ inline T shfl_xor32(T v) { return CX<T>::make(swap32(v.x), swap32(v.y)); }
 
 template <typename T, int NT> struct GeppLds {
     T rows[2][NT / 64][TW];
@@ -148,10 +159,10 @@ __device__ __attribute__((always_inline)) int reg_gepp(RowH<T> &x, int nr, int n
             const R inv = d != (R)0 ? (R)1 / d : (R)0;
             const T l = d != (R)0 ? tmul(a, CX<T>::make(pv.x * inv, -pv.y * inv)) : a;
             if (h == hk) x.set(jk, l);
-            else if (hk == 0) x.set(jk, tsub(x.get(jk), tmul(l, prow[2 * jk + 1])));   // column k + 1 of half 1
+            else if (hk == 0) x.set(jk, tfms(x.get(jk), l, prow[2 * jk + 1]));   // column k + 1 of half 1
             auto upd = [&](auto jc) __attribute__((always_inline)) {
                 constexpr int j = decltype(jc)::value;
-                x.set(j, tsub(x.get(j), tmul(l, prow[2 * j + h])));
+                x.set(j, tfms(x.get(j), l, prow[2 * j + h]));
             };
             Unroll<jk + 1, TH>::run(upd);
         }
