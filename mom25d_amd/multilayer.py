@@ -145,19 +145,43 @@ class LayerStack:
         return max([abs(l.er_c) for l in self.layers] + ([abs(self.top_er)] if self.top == 'open' else []) +
                    ([abs(self.bottom_er)] if self.bottom == 'open' else []))
 
-    # ---- the spectral network on the device (kr: complex128 tensor) ----------------------------------------
+    def key(self):
+        """The parameters the spectral functions depend on (at this stack's frequency)."""
+        return (self.top, self.bottom, complex(self.top_er), complex(self.bottom_er),
+                tuple((complex(l.er_c), float(l.h)) for l in self.layers))
+
+    # ---- the spectral network (kr: a complex128 device tensor, or a NumPy array for a few host points) ------
     def _network(self, kr, w, mode):
+        """The layers' admittances and transfer factors at kr. On the device the result is kept on the kr tensor
+        for the stack's parameters, w and the mode: every table (and the via kernels) of a frequency shares its
+        contour, so each network is evaluated once per frequency rather than once per interface pair."""
+        xp = torch if isinstance(kr, torch.Tensor) else np
+        key = (float(w), mode) + self.key()
+        cache = None
+        if xp is torch:
+            cache = getattr(kr, '_m25x_networks', None)
+            if cache is None:
+                cache = {}
+                kr._m25x_networks = cache
+            if key in cache:
+                return cache[key]
+        out = self._network_eval(kr, w, mode, xp)
+        if cache is not None:
+            cache[key] = out
+        return out
+
+    def _network_eval(self, kr, w, mode, xp):
         k0, N = w / C0, self.N
 
         def y0(e):
-            kz = -1j * torch.sqrt(kr * kr - k0 * k0 * e)
+            kz = -1j * xp.sqrt(kr * kr - k0 * k0 * e)
             return (kz / (w * MU0) if mode == 'h' else w * EPS0 * e / kz), kz
         Y0, tn, sc = [None] * (N + 1), [None] * (N + 1), [None] * (N + 1)
         for j in range(1, N + 1):
             Y0[j], kz = y0(self.eps(j))
             th = kz * self.layers[j - 1].h
-            E = torch.exp(-2j * th)
-            tn[j], sc[j] = -1j * (1 - E) / (1 + E), 2 * torch.exp(-1j * th) / (1 + E)
+            E = xp.exp(-2j * th)
+            tn[j], sc[j] = -1j * (1 - E) / (1 + E), 2 * xp.exp(-1j * th) / (1 + E)
         Yd = [None] * (N + 1)
         if self.bottom == 'pec':
             Yd[1], kfirst = -1j * Y0[1] / tn[1], 2
@@ -176,7 +200,7 @@ class LayerStack:
 
     def voltage(self, kr, w, o, s, mode):
         if self.is_pec(o) or self.is_pec(s):
-            return torch.zeros_like(kr)
+            return torch.zeros_like(kr) if isinstance(kr, torch.Tensor) else np.zeros_like(kr)
         Y0, tn, sc, Yd, Yu = self._network(kr, w, mode)
         v = 1.0 / (Yu[s] + Yd[s])
         for k in range(s + 1, o + 1):
@@ -191,10 +215,9 @@ class LayerStack:
         return Vh / (1j * w), 1j * w * (Ve - Vh) / (kr * kr)
 
     def spectral1(self, k, w, o, s):
-        """The spectral kernels at one real kr (a Python complex pair)."""
-        kt = torch.tensor([complex(k)], dtype=torch.complex128, device=dv.device())
-        ga, gq = self.spectral(kt, w, o, s)
-        return complex(ga[0].item()), complex(gq[0].item())
+        """The spectral kernels at one real kr (a Python complex pair), evaluated on the host."""
+        ga, gq = self.spectral(np.array([complex(k)]), w, o, s)
+        return complex(ga[0]), complex(gq[0])
 
 
 def sommerfeld_contour(f, stack, rho_max, K=None, nq=8):
@@ -215,6 +238,24 @@ def sommerfeld_contour(f, stack, rho_max, K=None, nq=8):
     s = np.where(t < t1, alpha * np.sin(np.pi * t / t1), 0.0)
     ds = np.where(t < t1, alpha * np.pi / t1 * np.cos(np.pi * t / t1), 0.0)
     return t + 1j * s, wt * (1 + 1j * ds), K
+
+
+_CONTOURS = {}
+
+
+def device_contour(f, stack, rho_max):
+    """(kr, dkr, K) of sommerfeld_contour on the device, kept for the last few (f, stack, rho_max): the tables and
+    the via kernels of a frequency share one contour tensor, and with it the spectral networks kept on it."""
+    st = stack.at(f)
+    key = (float(f), float(rho_max), st.key(), torch.cuda.current_device())
+    c = _CONTOURS.get(key)
+    if c is None:
+        kr_h, dk_h, K = sommerfeld_contour(f, st, rho_max)
+        c = (dv.on_dev(kr_h, dv.c128), dv.on_dev(dk_h, dv.c128), K)
+        if len(_CONTOURS) >= 4:
+            _CONTOURS.clear()
+        _CONTOURS[key] = c
+    return c
 
 
 _SPLINES = {}
@@ -312,8 +353,7 @@ def green_tables(f, stack, layers, rho_max, dmax):
     """{(a, b): GreenTableML} for every pair a <= b of metal interfaces (sim.py:736-745); the tables of one
     frequency share its contour (uploaded once)."""
     st = stack.at(f)
-    kr_h, dk_h, K = sommerfeld_contour(f, st, rho_max)
-    contour = (dv.on_dev(kr_h, dv.c128), dv.on_dev(dk_h, dv.c128), K)
+    contour = device_contour(f, stack, rho_max)
     out = {}
     for i, a in enumerate(layers):
         for b in layers[i:]:

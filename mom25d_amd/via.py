@@ -12,7 +12,7 @@ from scipy.special import kv
 
 from .base import C0, MU0
 from . import device as dv
-from .multilayer import hankel, resample, sommerfeld_contour, spline_operator
+from .multilayer import device_contour, hankel, resample, spline_operator
 
 
 class TMNetwork:
@@ -82,9 +82,8 @@ class ViaKernels:
 
     def __init__(self, f, stack, spans, interfaces, rho_max, nrho=200):
         w = 2 * np.pi * f
+        kr, dk, _ = device_contour(f, stack, rho_max)            # the contour of the frequency's tables
         stack = stack.at(f)
-        kr_h, dk_h, _ = sommerfeld_contour(f, stack, rho_max)
-        kr, dk = dv.on_dev(kr_h, dv.c128), dv.on_dev(dk_h, dv.c128)
         net = TMNetwork(stack, kr, w)
         self.f, self.st, self.w, self.kr, self.dk, self.net = f, stack, w, kr, dk, net
         self._gq = {}
