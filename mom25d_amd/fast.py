@@ -23,6 +23,7 @@ from .linalg import gmres_batched
 HYBRID_RETRY = 1.25
 MIXED_RETRY = HYBRID_RETRY
 MIXED_MAX_ITER = 30
+POLISH = 10.0
 FRESH_STALL = (4, 0.1)
 REUSE_ITER = (8, 60)
 REUSE_KEEP = 4
@@ -95,12 +96,23 @@ class FastOperator:
 
     # ---- solve ---------------------------------------------------------------------------------------------
     def _refine(self, A, B, M, restart, maxiter, stall=None):
-        """GMRES to mixed_tol: (Y, converged, largest relative residual) (fast.py:658-663)."""
+        """GMRES to mixed_tol: (Y, converged, largest relative residual) (fast.py:658-663). Whether the answer is
+        accepted (a residual within 100 x mixed_tol) decides the path as in the reference; an accepted answer whose
+        residual is still above POLISH x mixed_tol gets one more restart cycle from itself (S is then within ~3e-5 of
+        the double-precision one rather than ~3e-4)."""
         Y, self.iters = gmres_batched(A, B, M=M, rtol=self.mixed_tol, restart=restart, maxiter=maxiter, stall=stall)
-        rel = (torch.linalg.vector_norm(B - A(Y), dim=0) /
-               torch.clamp(torch.linalg.vector_norm(B, dim=0), min=1e-300)).cpu().numpy()
+        rel = self._residual(A, B, Y)
         ok = bool(np.isfinite(rel).all() and (rel <= 100 * self.mixed_tol).all())
+        if ok and len(rel) and rel.max() > POLISH * self.mixed_tol:
+            Y, more = gmres_batched(A, B, M=M, x0=Y, rtol=self.mixed_tol, restart=restart, maxiter=1)
+            self.iters = [a + b for a, b in zip(self.iters, more)]
+            rel = self._residual(A, B, Y)
         return Y, ok, float(np.max(rel)) if len(rel) else 0.0
+
+    @staticmethod
+    def _residual(A, B, Y):
+        return (torch.linalg.vector_norm(B - A(Y), dim=0) /
+                torch.clamp(torch.linalg.vector_norm(B, dim=0), min=1e-300)).cpu().numpy()
 
     def _reusable(self, f):
         if self.sim.reuse == 'off' or not self._kept:
