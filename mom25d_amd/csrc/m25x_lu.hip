@@ -16,8 +16,9 @@
 // grid barrier (k_tslu), its row moves on every column (k_apply_moves), then L21 = A21 U11^-1 by substitution,
 // U12 = L11^-1 A12 by substitution and A22 -= L21 U12 on the matrix cores (complex64 with FP64 accumulation: one
 // rounding per update).
-// The solves (getrs): one launch per triangle; a chain workgroup solves the 64-row diagonal blocks in order while
-// helper workgroups accumulate the products with earlier blocks, values handed over as self-marking words.
+// The solves (getrs): one launch per triangle, one workgroup per 64-row diagonal block; each accumulates the
+// products of its rows with the earlier blocks as their values appear, then solves its own block and hands its
+// values on as self-marking words.
 #include <mutex>
 #include "m25x_lu_dev.h"
 
@@ -541,368 +542,198 @@ __device__ inline bool wave_all(bool c) { return __builtin_amdgcn_ballot_w64(!c)
 // ---- the triangular solve ----------------------------------------------------------------------------------
 constexpr int TB = 512;                  // threads of a solve workgroup: 8 waves
 constexpr int NWV = TB / 64;
-constexpr int TE = SB * SB / TB;         // entries of a 64 x 64 tile a helper thread takes: 8
-constexpr int TL = 2 * TE;               // entries of a tile a loader thread stages: 16
-constexpr int HD = 2;                    // the chain applies y_q to blocks q + 1 and q + 2; helpers the rest
-constexpr int TRSV_LEAD = 3;             // blocks ahead of the chain its prefetchers read
-constexpr int TRSV_NPF = 4;              // prefetch workgroups
+constexpr int TE = SB * SB / TB;         // entries of a 64 x 64 tile a thread takes: 8
+template <typename T> struct TrsvDepth { static constexpr int D = 3; };  // tiles a workgroup keeps in flight
+template <> struct TrsvDepth<double2> { static constexpr int D = 2; };      // ahead of the values they meet
 template <typename T> struct TrsvMaxRhs { static constexpr int NR = 8; };    // right-hand sides a launch takes
 template <> struct TrsvMaxRhs<double2> { static constexpr int NR = 4; };
 
 // Unit-lower (FWD) or unit-upper substitution of NW right-hand sides, lane i row i, with the block's operator Ms
 // in LDS (rows pre-scaled to a unit diagonal; the diagonal, the other triangle and the padding zero): step k
-// broadcasts y_k and every lane subtracts its multiple, the zeros leaving the finished rows alone. The same
-// broadcast feeds the next block: a -= Cs[lane][k] y_k.
+// broadcasts y_k and every lane subtracts its multiple, the zeros leaving the finished rows alone.
 template <bool FWD, int NW, typename T>
-__device__ __attribute__((always_inline)) void subst_fused(T (&t)[NW], T (&a)[NW], const T (*Ms)[SB + 1],
-                                                          const T (*Cs)[SB + 1], int lane)
+__device__ __attribute__((always_inline)) void subst(T (&t)[NW], const T (*Ms)[SB + 1], int lane)
 {
 #pragma unroll
     for (int s0 = 0; s0 < SB; s0 += 8) {
-        T mk[8], mc[8];
+        T mk[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int k = FWD ? s0 + j : SB - 1 - s0 - j;
-            mk[j] = Ms[lane][k];
-            mc[j] = Cs[lane][k];
-        }
+        for (int j = 0; j < 8; ++j) mk[j] = Ms[lane][FWD ? s0 + j : SB - 1 - s0 - j];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int k = FWD ? s0 + j : SB - 1 - s0 - j;
 #pragma unroll
-            for (int w = 0; w < NW; ++w) {
-                const T y = readlane_c(t[w], k);
-                t[w] = cfms(t[w], mk[j], y);
-                a[w] = cfms(a[w], mc[j], y);
-            }
+            for (int w = 0; w < NW; ++w) t[w] = cfms(t[w], mk[j], readlane_c(t[w], k));
         }
     }
 }
 
-// One launch of nb + 1 workgroups of 512 threads solves one triangle for nrhs <= NR right-hand sides.
-//
-// Workgroup 0, the chain, solves the 64-row diagonal blocks in order, in three barriers a block (A, B, C).
-// Waves 0-3, its solvers (wave w takes right-hand sides w and, for NR = 8, w + 4; lane i row i): t = src_q - P_q
-// + a_q, then the unit-diagonal substitution, which also applies y_q to block q + 1 (a_{q+1}); y_q published to dst
-// and left in LDS. Waves 4-7, its loaders (thread: row lane, sixteen columns): at A they write block q's tiles
-// (the diagonal block's strict triangle, scaled; the coupling to block q + 1) from registers into LDS; after B
-// they issue the loads of block q + 1's tiles and of Op_{q+2, q}; after C, with y_q in LDS, they apply y_q to
-// block q + 2 and leave their partial sums for the solvers. No wait on device memory sits on the solvers' path but
-// for P_q, which they ask for a block ahead.
-//
-// Workgroup 1 + q, a helper, accumulates for block q the products with the blocks published before q - 2,
-// P_q = sum_a Op_qa y_a, as each y_a appears in dst, and publishes P_q: two blocks' time to do it. y (dst) and P
-// travel as self-marking values (put / take): a reader spins on the value itself.
+// One launch of nb workgroups of 512 threads solves one triangle for nrhs <= NR right-hand sides. Workgroup q
+// owns the q-th 64-row diagonal block in solve order (the chain passes from workgroup to workgroup):
+//  - at its start it stages its diagonal block's operator in LDS (strict triangle, rows scaled by 1 / U_ii) and
+//    reads its right-hand sides;
+//  - it then meets the earlier blocks a in order: thread (row lane, columns 8 wv + u of block a) holds its
+//    entries of Op_qa, loaded TrsvDepth blocks ahead, and adds Op_qa y_a to its partial sums once y_a is out;
+//  - after the last one (a = q - 1) the partial sums meet in LDS, waves 0-3 form the block's right-hand side
+//    (wave w takes right-hand sides w and, for NR = 8, w + 4; lane i row i), substitute, and publish y_q.
+// y travels as self-marking values (put / take) in dst: a reader spins on the value itself, tightly when the block
+// is one of the two before its own (the chain waits on it), with growing naps otherwise. Workgroup q waits only on
+// workgroups before it, which start no later, so the launch makes progress with any number of them resident.
 //
 // rdg: 1 / U_ii (read for the kinds with U only: L is unit). src: n x nrhs (leading dimension lds); dst: n x nrhs
-// (leading dimension nrhs) and P (nb x 64 x nrhs) filled with UNSET words beforehand. abort: raised if a wait
-// exceeds its bound (the result is then garbage, not a hang). NR: 4 (nrhs <= 4) or 8.
+// (leading dimension nrhs) filled with UNSET words beforehand. abort: raised if a wait exceeds its bound (the result
+// is then garbage, not a hang). NR: 4 (nrhs <= 4) or 8.
 template <typename T, int NR>
 __global__ __launch_bounds__(TB) void k_trsv(const T *LU, long lda, long n, const T *rdg, const T *src, long lds,
-                                             T *dst, int nrhs, int kind, T *P, int *abort)
+                                             T *dst, int nrhs, int kind, int *abort)
 {
     constexpr int NW = NR / 4;               // right-hand sides a solver wave takes
-    __shared__ T Ms[SB][SB + 1];             // chain: the diagonal operator; helpers: the slices' partial sums
-    __shared__ T Cs[SB][SB + 1];             // chain: Op_{q+1, q}; helpers: y_a by slice
-    __shared__ T rds[SB];                    // chain: 1 / U_ii of the block's rows (one for the unit triangles)
-    __shared__ T ys[NR][SB];                 // chain: y_q
-    __shared__ T lp[4][NR][SB];              // chain: the loaders' partial sums Op_{q+2, q} y_q
-    static_assert(NWV * NR * SB <= SB * (SB + 1) && NWV * 64 <= SB * (SB + 1), "helper arrays fit");
+    constexpr int D = TrsvDepth<T>::D;
+    __shared__ T Ms[SB][SB + 1];             // the diagonal block's operator
+    __shared__ T red[NWV][NR][SB];           // the waves' partial sums
+    __shared__ T yb[NWV][64];                // each wave's y values of the block it meets
     const int nb = (int)((n + SB - 1) / SB);
+    const int q = blockIdx.x;
+    if (q >= nb) return;
     const bool fwd = kind_fwd(kind), rowmajor = kind <= 1, unit = kind == 1 || kind == 2;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const T zero = CX<T>::make(0, 0), one = CX<T>::make(1, 0);
-    auto order = [&](int q) { return fwd ? q : nb - 1 - q; };
-    auto spin = [&](long &spins) {
-        __builtin_amdgcn_s_sleep(1);
-        if (++spins > (1L << 24)) {
-            __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return false;
-        }
-        return true;
-    };
+    auto order = [&](int p) { return fwd ? p : nb - 1 - p; };
     auto opp = [&](long gi, long gk) { return rowmajor ? LU + gi * lda + gk : LU + gk * lda + gi; };
     const long kstep = rowmajor ? 1 : lda;   // address step along a row of the operator
+    const int b = order(q);
+    const long r0 = (long)b * SB;
+    const int m = (int)std::min((long)SB, n - r0);          // rows of this block inside the matrix
 
-    if (blockIdx.x > (unsigned)nb) {
-        // ---- prefetcher: workgroups past the helpers whose index is a multiple of 8 run on the chain's XCD
-        // (workgroups are dealt to the 8 XCDs in turn); they read the tiles the chain's loaders will ask for a few
-        // blocks ahead, so that those loads hit that XCD's L2 instead of waiting on HBM
-        const int first = (nb + 1 + 7) & ~7;
-        if ((blockIdx.x & 7) || (int)blockIdx.x < first) return;
-        const int j = (blockIdx.x - first) >> 3;
-        const int stride = (gridDim.x - first + 7) >> 3;
-        float sum = 0.f;
-        for (int q = j; q < nb; q += stride) {
-            // the chain loads block q's tiles during step q - 1: start once y of block q - TRSV_LEAD is out
-            if (q >= TRSV_LEAD && lane == 0 && wv == 0) {
-                const T *w0 = dst + (long)order(q - TRSV_LEAD) * SB * nrhs;
-                T v;
-                long spins = 0;
-                int nap = 1;
-                while (!take(w0, v)) {
-                    for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(2);
-                    nap = nap < 16 ? 2 * nap : 16;
-                    if (++spins > (1L << 22)) break;
-                }
-            }
-            __syncthreads();
-            const int b = order(q);
-            const long r0 = (long)b * SB;
-            // the diagonal block, its coupling to block q + 1, and Op_{q+1, q-1}: rows (8 wv + u) of a row-major
-            // operator, columns of a column-major one, lane along the contiguous index
-            auto touch = [&](long rr0, long cc0) __attribute__((always_inline)) {
-                const T *base = opp(rr0, cc0);
-                const int lim = (int)std::min((long)SB, n - (rowmajor ? rr0 : cc0)) - 1;     // last slow index
-                const int f = (int)std::min((long)lane, n - 1 - (rowmajor ? cc0 : rr0));
+    // the diagonal block: thread (rows 8 wv + u, column lane) of a row-major operator, (row lane, columns 8 wv + u)
+    // of a column-major one, so that every load reads whole lines; indices clamped into the block, the entries
+    // outside it or outside the strict triangle zeroed when staged
+    T md[TE], mr[TE];
 #pragma unroll
-                for (int u = 0; u < TE; ++u) sum += base[(long)std::min(wv * TE + u, lim) * lda + f].x;
-            };
-            touch(r0, r0);
-            if (q + 1 < nb) {
-                const long r1 = (long)order(q + 1) * SB;
-                touch(r1, r0);
-                if (q >= 1) touch(r1, (long)order(q - 1) * SB);
-            }
-        }
-        if (sum == -1.2345e-30f) __hip_atomic_store(abort + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return;
+    for (int u = 0; u < TE; ++u) {
+        const int i = rowmajor ? wv * TE + u : lane, k = rowmajor ? lane : wv * TE + u;
+        const long gi = r0 + std::min(i, m - 1), gk = r0 + std::min(k, m - 1);
+        md[u] = *opp(gi, gk);
+        mr[u] = rdg[gi];                     // read either way: a select of values, not of addresses
     }
-    if (blockIdx.x > 0) {
-        // ---- helper of the q-th block -------------------------------------------------------------------------
-        const int q = blockIdx.x - 1;
-        if (q <= HD) return;
-        T (*red)[NR][SB] = reinterpret_cast<T (*)[NR][SB]>(&Ms[0][0]);
-        const int b = order(q), nbulk = q - HD;
-        // thread (row lane, columns 8 wv + u of each block a); a row past the matrix reads the last row (its sums
-        // are never published)
-        auto load_htile = [&](T (&m)[TE], int a) __attribute__((always_inline)) {
-            const long gi = std::min((long)b * SB + lane, n - 1), k0 = (long)a * SB + wv * TE;
-            const T *p = opp(gi, k0);
-            if (k0 + TE <= n) {
-#pragma unroll
-                for (int u = 0; u < TE; ++u) m[u] = p[u * kstep];
-            } else {                                 // columns past the matrix meet zero y
-                const int ul = (int)(n - 1 - k0);
-#pragma unroll
-                for (int u = 0; u < TE; ++u) m[u] = p[(u < ul ? u : ul) * kstep];
-            }
-        };
-        T s[NR];
-#pragma unroll
-        for (int r = 0; r < NR; ++r) s[r] = zero;
-        // lane l < 8 NR of wave wv fetches y (row 8 wv + l / NR of block a, right-hand side l % NR; zero for the
-        // right-hand sides past nrhs)
-        const int lu_ = lane / NR, lr = lane % NR;
-        T *yb = &Cs[0][0] + wv * 64;         // this wave's 64 values in LDS
-        auto step = [&](int pp, const T (&mc)[TE], T (&mn)[TE]) __attribute__((always_inline)) {
-            const int a = order(pp);
-            const long gk = (long)a * SB + wv * TE + lu_;
-            const bool need = lane < TE * NR && lr < nrhs && gk < n;
-            // one lane watches the first value of the wave's slice, backing off while the chain is far away (a
-            // lighter load on the memory system than every lane polling); then every lane reads its own
-            if (lane == 0 && (long)a * SB + wv * TE < n) {         // (a slice past the matrix needs nothing)
-                const T *w0 = dst + ((long)a * SB + wv * TE) * nrhs;
-                T v;
-                long spins = 0;
-                int nap = 1;
-                while (!take(w0, v)) {
-                    for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(2);
-                    nap = nap < 16 ? 2 * nap : 16;
-                    if (++spins > (1L << 22)) {
-                        __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        break;
-                    }
-                }
-            }
-            T yv = zero;
-            bool ok = !need || take(dst + gk * nrhs + lr, yv);
-            if (pp + 1 < nbulk) load_htile(mn, order(pp + 1));      // in flight behind y_a
-            long spins = 0;
-            while (!wave_all(ok)) {
-                if (!spin(spins)) break;
-                if (!ok) ok = take(dst + gk * nrhs + lr, yv);
-            }
-            if (!need) yv = zero;
-            yb[lane] = yv;
-            __builtin_amdgcn_wave_barrier();
-#pragma unroll
-            for (int u = 0; u < TE; ++u)
-#pragma unroll
-                for (int r = 0; r < NR; ++r) s[r] = tadd(s[r], tmul(mc[u], yb[u * NR + r]));
-            __builtin_amdgcn_wave_barrier();
-        };
-        T m0[TE], m1[TE];
-        load_htile(m0, order(0));
-        for (int pp = 0; pp < nbulk; pp += 2) {
-            step(pp, m0, m1);
-            if (pp + 1 < nbulk) step(pp + 1, m1, m0);
-        }
-#pragma unroll
-        for (int r = 0; r < NR; ++r) red[wv][r][lane] = s[r];
-        __syncthreads();
-        if (wv < nrhs && (long)b * SB + lane < n) {
-            T v = zero;
-#pragma unroll
-            for (int sl = 0; sl < NWV; ++sl) v = tadd(v, red[sl][wv][lane]);
-            put(P + ((size_t)b * SB + lane) * nrhs + wv, v);
-        }
-        return;
-    }
-
-    // ---- the chain --------------------------------------------------------------------------------------------
-    if (wv >= 4) {
-        // ---- loaders: thread (rows 16 lw + u, column lane) of a row-major operator, (row lane, columns 16 lw + u)
-        // of a column-major one, so that every load instruction reads whole lines. The loads are unconditional
-        // (indices clamped into the matrix) so that they all issue at once; entries outside the matrix or the
-        // diagonal block's strict triangle are zeroed when staged.
-        const int lw = __builtin_amdgcn_readfirstlane(wv - 4);
-        auto ij = [&](int u, int &i, int &k) __attribute__((always_inline)) {
-            if (rowmajor) { i = lw * TL + u; k = lane; }
-            else { i = lane; k = lw * TL + u; }
-        };
-        T md[TL], mc[TL], mx[TL];
-        // tile (rows of block rb, columns of block cb): element (slow index 16 lw + u, fast index lane) at
-        // base + slow * lda + fast. Past the matrix the indices are clamped: those entries meet only zero y or are
-        // zeroed when staged.
-        const int ld32 = (int)lda;           // offsets inside a tile's 64 x lda span fit 32 bits
-        auto load = [&](T (&m)[TL], int rb, int cb) __attribute__((always_inline)) {
-            const long r0 = (long)rb * SB, c0 = (long)cb * SB;
-            const T *base = opp(r0, c0) + (long)lw * TL * lda;    // uniform across the wave
-            if (r0 + SB <= n && c0 + SB <= n) {
-#pragma unroll
-                for (int u = 0; u < TL; ++u) m[u] = base[u * ld32 + lane];
-            } else {
-                const int rl = (int)(n - 1 - r0), cl = (int)(n - 1 - c0);
-                const int sl = (rowmajor ? rl : cl) - lw * TL, f = std::min(lane, rowmajor ? cl : rl);
-#pragma unroll
-                for (int u = 0; u < TL; ++u) m[u] = base[std::min(u, sl) * ld32 + f];
-            }
-        };
-        // entry u of the diagonal block b's operator kept: inside the matrix and in the strict triangle
-        auto keep = [&](int u, int b) __attribute__((always_inline)) {
-            int i, k;
-            ij(u, i, k);
-            const int m = (int)std::min((long)SB, n - (long)b * SB);
-            return i < m && k < m && (fwd ? k < i : k > i);
-        };
-        load(md, order(0), order(0));
-        if (nb > 1) load(mc, order(1), order(0));
-        for (int q = 0; q < nb; ++q) {
-            const int b = order(q), b1 = q + 1 < nb ? order(q + 1) : -1, b2 = q + 2 < nb ? order(q + 2) : -1;
-            __syncthreads();                             // A: the solvers are done with the last tiles
-#pragma unroll
-            for (int u = 0; u < TL; ++u) {
-                int i, k;
-                ij(u, i, k);
-                Ms[i][k] = csel(keep(u, b), tmul(md[u], rds[i]), zero);
-                Cs[i][k] = b1 >= 0 ? mc[u] : zero;
-            }
-            __syncthreads();                             // B: the tiles are in LDS
-            if (b1 >= 0) {
-                load(md, b1, b1);
-                if (b2 >= 0) load(mc, b2, b1);
-            }
-            if (b2 >= 0) load(mx, b2, b);                // Op_{q+2, q}
-            __syncthreads();                             // C: y_q is in LDS; Ms and Cs are free until A
-            T part[NR];
-#pragma unroll
-            for (int r = 0; r < NR; ++r) part[r] = zero;
-            if (b2 >= 0) {
-                if (!rowmajor) {                         // lane's row, columns 16 lw + u
-#pragma unroll
-                    for (int u = 0; u < TL; ++u)
-#pragma unroll
-                        for (int r = 0; r < NR; ++r) part[r] = tadd(part[r], tmul(mx[u], ys[r][lw * TL + u]));
-#pragma unroll
-                    for (int r = 0; r < NR; ++r) lp[lw][r][lane] = part[r];
-                } else {
-                    // the wave's sixteen rows through its own rows of Ms: lane (row 16 lw + lane / 4, columns
-                    // 16 (lane % 4) + j)
-#pragma unroll
-                    for (int u = 0; u < TL; ++u) Ms[lw * TL + u][lane] = mx[u];
-                    __builtin_amdgcn_wave_barrier();
-                    const int i = lw * TL + (lane >> 2), c = lane & 3;
-#pragma unroll
-                    for (int j = 0; j < TL; ++j) {
-                        const T x = Ms[i][c * TL + j];
-#pragma unroll
-                        for (int r = 0; r < NR; ++r) part[r] = tadd(part[r], tmul(x, ys[r][c * TL + j]));
-                    }
-#pragma unroll
-                    for (int r = 0; r < NR; ++r) lp[c][r][i] = part[r];
-                }
-            } else {
-#pragma unroll
-                for (int r = 0; r < NR; ++r) lp[lw][r][lane] = zero;
-            }
-        }
-        return;
-    }
-    // ---- solvers ------------------------------------------------------------------------------------------------
-    T bx[NW], px[NW], acc[NW];                   // lane's row: source, P, - the chain's products (block q)
-    T rdl = one;                                 // 1 / U_ii of the lane's row of the next block
-    bool act[NW];
+    // the solver waves' right-hand sides and 1 / U_ii of the lane's row (rows past n: zeroed where used)
+    T bx[NW], rdl;
     int rr[NW];
+    {
+        const long gi = r0 + std::min(lane, m - 1);
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            rr[w] = (wv & 3) + 4 * w;
+            bx[w] = src[gi * lds + std::min(rr[w], nrhs - 1)];
+        }
+        rdl = csel(unit, one, rdg[gi]);
+    }
+
+    // the earlier blocks' entries: thread (row lane, columns 8 wv + u of block a); a row past the matrix reads the
+    // last row (its sums are never used), columns past it meet zero y
+    auto load_tile = [&](T (&mt)[TE], int a) __attribute__((always_inline)) {
+        const long gi = r0 + std::min(lane, m - 1), k0 = (long)a * SB + wv * TE;
+        const T *p = opp(gi, k0);
+        if (k0 + TE <= n) {
+#pragma unroll
+            for (int u = 0; u < TE; ++u) mt[u] = p[u * kstep];
+        } else {                             // columns from n on read column n - 1 (ul < 0 past the matrix)
+            const int ul = (int)(n - 1 - k0);
+#pragma unroll
+            for (int u = 0; u < TE; ++u) mt[u] = p[(u < ul ? u : ul) * kstep];
+        }
+    };
+    auto stage = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int u = 0; u < TE; ++u) {
+            const int i = rowmajor ? wv * TE + u : lane, k = rowmajor ? lane : wv * TE + u;
+            const bool in = i < m && k < m && (fwd ? k < i : k > i);
+            Ms[i][k] = csel(in, csel(unit, md[u], tmul(md[u], mr[u])), zero);
+        }
+    };
+    // complex128 stages the diagonal block before its first tiles are asked for (registers), complex64 after
+    if constexpr (sizeof(T) == 16) stage();
+    T m0[TE], m1[TE], m2[TE];
+    if (q > 0) load_tile(m0, order(0));
+    if (q > 1) load_tile(m1, order(1));
+    if (D > 2 && q > 2) load_tile(m2, order(2));
+    if constexpr (sizeof(T) != 16) stage();
+
+    T s[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) s[r] = zero;
+    // lane l < 8 NR of wave wv takes y of row 8 wv + l / NR of block a, right-hand side l % NR (zero for the
+    // right-hand sides past nrhs)
+    const int lu_ = lane / NR, lr = lane % NR;
+    T *ybw = yb[wv];
+    auto step = [&](int p, T (&mt)[TE]) __attribute__((always_inline)) {
+        const int a = order(p);
+        const long g0 = (long)a * SB + wv * TE, gk = g0 + lu_;
+        const bool need = lane < TE * NR && lr < nrhs && gk < n;
+        if (lane == 0 && g0 < n) {           // one lane watches the slice's first value (a slice past n needs none)
+            const T *w0 = dst + g0 * nrhs;
+            const bool near = q - p <= 2;
+            T v;
+            long spins = 0;
+            int nap = 1;
+            while (!take(w0, v)) {
+                if (near) __builtin_amdgcn_s_sleep(1);
+                else {
+                    for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(2);
+                    nap = nap < 8 ? 2 * nap : 8;
+                }
+                if (++spins > (1L << 22)) {
+                    __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        T yv = zero;
+        bool ok = !need || take(dst + gk * nrhs + lr, yv);
+        long spins = 0;
+        while (!wave_all(ok)) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > (1L << 22)) {
+                __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            if (!ok) ok = take(dst + gk * nrhs + lr, yv);
+        }
+        if (!need) yv = zero;
+        ybw[lane] = yv;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int u = 0; u < TE; ++u) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) s[r] = tadd(s[r], tmul(mt[u], ybw[u * NR + r]));
+            if constexpr (sizeof(T) == 16) __builtin_amdgcn_sched_barrier(0);     // y read as it is used
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (p + D < q) load_tile(mt, order(p + D));
+    };
+    for (int p = 0; p < q; p += D) {
+        step(p, m0);
+        if (p + 1 < q) step(p + 1, m1);
+        if (D > 2 && p + 2 < q) step(p + 2, m2);
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) red[wv][r][lane] = s[r];
+    __syncthreads();
+    if (wv >= 4) return;
+    const bool inrow = lane < m;
+    T t[NW];
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
-        rr[w] = wv + 4 * w;
-        act[w] = rr[w] < nrhs;
-        bx[w] = px[w] = acc[w] = zero;
+        T v = zero;
+#pragma unroll
+        for (int sl = 0; sl < NWV; ++sl) v = tadd(v, red[sl][rr[w]][lane]);
+        t[w] = csel(inrow && rr[w] < nrhs, tmul(tsub(bx[w], v), rdl), zero);
     }
-    auto issue_src = [&](int q) __attribute__((always_inline)) {      // source values and 1 / U_ii of block q
-        const long gi = std::min((long)order(q) * SB + lane, n - 1);    // rows past n: zeroed where used
+    if (fwd) subst<true, NW>(t, Ms, lane);
+    else subst<false, NW>(t, Ms, lane);
 #pragma unroll
-        for (int w = 0; w < NW; ++w) bx[w] = src[gi * lds + std::min(rr[w], nrhs - 1)];
-        const T rv = rdg[gi];                   // read either way: a select of values, not of addresses
-        rdl = csel(unit, CX<T>::make(1, 0), rv);
-    };
-    auto has_p = [&](int q) { return q > HD; };                       // P of block q comes from a helper
-    issue_src(0);
-    if (wv == 0) rds[lane] = (long)order(0) * SB + lane < n ? rdl : zero;
-    for (int q = 0; q < nb; ++q) {
-        const long i0 = (long)order(q) * SB;
-        const bool inrow = i0 + lane < n;
-        T t[NW];
-#pragma unroll
-        for (int w = 0; w < NW; ++w) t[w] = csel(inrow && act[w], tmul(tadd(tsub(bx[w], px[w]), acc[w]), rdl), zero);
-        __syncthreads();                                 // A
-        // block q + 1: its source, 1 / U_ii, P (asked for now, checked after C), the loaders' partial sums
-        const long g1 = (long)order(q + 1 < nb ? q + 1 : q) * SB + lane;
-        const bool p1 = q + 1 < nb && has_p(q + 1) && g1 < n;
-        if (q + 1 < nb) issue_src(q + 1);
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            px[w] = p1 && act[w] ? fetch(P + g1 * nrhs + rr[w]) : zero;
-            T l = zero;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) l = tadd(l, lp[j][w * 4 + (wv & 3)][lane]);
-            acc[w] = q >= 1 ? tsub(zero, l) : zero;      // Op_{q+1, q-1} y_{q-1}, from the loaders
-        }
-        __syncthreads();                                 // B: block q's tiles in LDS
-        if (fwd) subst_fused<true, NW>(t, acc, Ms, Cs, lane);
-        else subst_fused<false, NW>(t, acc, Ms, Cs, lane);
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            ys[w * 4 + (wv & 3)][lane] = t[w];           // zero for the right-hand sides past nrhs
-            if (act[w] && inrow) put(dst + (i0 + lane) * nrhs + rr[w], t[w]);
-        }
-        __syncthreads();                                 // C: y_q in LDS
-        if (q + 1 < nb) {
-            bool ok = true;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) ok = ok && (!p1 || !act[w] || is_set(px[w]));
-            long spins = 0;
-            while (!wave_all(ok)) {
-                if (!spin(spins)) break;
-                ok = true;
-#pragma unroll
-                for (int w = 0; w < NW; ++w)
-                    if (p1 && act[w] && !is_set(px[w])) ok = take(P + g1 * nrhs + rr[w], px[w]) && ok;
-            }
-            if (wv == 0) rds[lane] = (long)order(q + 1) * SB + lane < n ? rdl : zero;   // staged after the next A
-        }
-    }
+    for (int w = 0; w < NW; ++w)
+        if (rr[w] < nrhs && inrow) put(dst + (r0 + lane) * nrhs + rr[w], t[w]);
 }
 
 // 1 / U_ii for the substitutions with U and U^T
@@ -1050,12 +881,11 @@ int getrf_tp(T *A, long n, long lda, int *ipiv, int *perm, int *info, void *work
     return M25X_OK;
 }
 
-// Solve workspace: y of the two solves (W1, W2) and their partial sums (P1, P2), written as self-marking values over
-// UNSET words; the gathered right-hand sides (W0); 1 / U_ii; the permutation and the abort word.
+// Solve workspace: y of the two solves (W1, W2), written as self-marking values over UNSET words; the gathered
+// right-hand sides (W0); 1 / U_ii; the permutation and the abort word.
 template <typename T> size_t getrs_work(long n, long)
 {
-    const long nb = (n + SB - 1) / SB;
-    return sizeof(T) * (3 * (size_t)n * MAXRHS + 2 * (size_t)nb * SB * MAXRHS + (size_t)n) + sizeof(int) * ((size_t)n + 64);
+    return sizeof(T) * (3 * (size_t)n * MAXRHS + (size_t)n) + sizeof(int) * ((size_t)n + 64);
 }
 
 // ---- launch timing of the triangular solves (benchmarks; one host thread) -----------------------------------
@@ -1083,17 +913,14 @@ int tt_mark(hipStream_t s)
 
 template <typename T>
 int trsv_launch(hipStream_t s, int nb, const T *LU, long lda, long n, const T *rd, const T *src, long lds, T *dst,
-                int nrhs, int kind, T *P, int *abort)
+                int nrhs, int kind, int *abort)
 {
     if (int rc = tt_mark(s)) return rc;
-    // the chain, nb helpers, then up to the next multiple of 8 and TRSV_NPF groups of 8 (one prefetcher each)
-    const unsigned grid = (unsigned)(((nb + 1 + 7) & ~7) + 8 * TRSV_NPF);
     if (nrhs <= 4)
-        hipLaunchKernelGGL((k_trsv<T, 4>), dim3(grid), dim3(TB), 0, s, LU, lda, n, rd, src, lds, dst, nrhs, kind, P,
-                           abort);
+        hipLaunchKernelGGL((k_trsv<T, 4>), dim3(nb), dim3(TB), 0, s, LU, lda, n, rd, src, lds, dst, nrhs, kind, abort);
     else
-        hipLaunchKernelGGL((k_trsv<T, TrsvMaxRhs<T>::NR>), dim3(grid), dim3(TB), 0, s, LU, lda, n, rd, src, lds, dst,
-                           nrhs, kind, P, abort);
+        hipLaunchKernelGGL((k_trsv<T, TrsvMaxRhs<T>::NR>), dim3(nb), dim3(TB), 0, s, LU, lda, n, rd, src, lds, dst,
+                           nrhs, kind, abort);
     if (int rc = check_launch("getrs/trsv")) return rc;
     if (g_tt.on) {
         const double tri = kind_fwd(kind) ? 0.5 * n * (n - 1) : 0.5 * n * (n + 1);
@@ -1116,11 +943,11 @@ int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, 
     if (!work || wbytes < getrs_work<T>(n, ldb))
         return fail(M25X_EINVAL, "getrs: workspace of %zu bytes, %zu needed", wbytes, getrs_work<T>(n, ldb));
     const int nb = (int)((n + SB - 1) / SB);
-    const size_t wn = (size_t)n * MAXRHS, pn = (size_t)nb * SB * MAXRHS;
-    T *W1 = (T *)work, *W2 = W1 + wn, *P1 = W2 + wn, *P2 = P1 + pn, *W0 = P2 + pn, *rd = W0 + wn;
+    const size_t wn = (size_t)n * MAXRHS;
+    T *W1 = (T *)work, *W2 = W1 + wn, *W0 = W2 + wn, *rd = W0 + wn;
     int *perm = (int *)(rd + n);
     int *abort = perm + n;
-    const long words = (long)((2 * wn + 2 * pn) * sizeof(T) / 8);
+    const long words = (long)(2 * wn * sizeof(T) / 8);
     hipLaunchKernelGGL(k_fill_unset, dim3(blocks(words, 256)), dim3(256), 0, s, (unsigned long long *)W1, words);
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, abort, 1);
     hipLaunchKernelGGL(k_rdiag<T>, dim3(blocks(n, 256)), dim3(256), 0, s, LU, lda, n, rd);
@@ -1132,13 +959,13 @@ int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, 
     const unsigned pg = blocks(n * nrhs, 256);
     int rc;
     if (trans) {
-        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, rd, B, ldb, W1, nrhs, 0, P1, abort))) return rc;
-        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, rd, W1, nrhs, W2, nrhs, 1, P2, abort))) return rc;
+        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, rd, B, ldb, W1, nrhs, 0, abort))) return rc;
+        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, rd, W1, nrhs, W2, nrhs, 1, abort))) return rc;
         hipLaunchKernelGGL(k_permute<T>, dim3(pg), dim3(256), 0, s, (const T *)W2, (long)nrhs, B, ldb, n, nrhs, perm_in, 1);
     } else {
         hipLaunchKernelGGL(k_permute<T>, dim3(pg), dim3(256), 0, s, (const T *)B, ldb, W0, (long)nrhs, n, nrhs, perm_in, 0);
-        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, rd, W0, nrhs, W1, nrhs, 2, P1, abort))) return rc;
-        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, rd, W1, nrhs, W2, nrhs, 3, P2, abort))) return rc;
+        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, rd, W0, nrhs, W1, nrhs, 2, abort))) return rc;
+        if ((rc = trsv_launch<T>(s, nb, LU, lda, n, rd, W1, nrhs, W2, nrhs, 3, abort))) return rc;
         hipLaunchKernelGGL(k_permute<T>, dim3(pg), dim3(256), 0, s, (const T *)W2, (long)nrhs, B, ldb, n, nrhs,
                            (const int *)nullptr, 0);
     }
