@@ -5,12 +5,26 @@ the GPU (the FP64 Z product of m25x_zgemm_nk, the complex64 factor's solves of m
 on the device and is orthogonalised by classical Gram-Schmidt applied twice (the reference uses modified
 Gram-Schmidt: the same basis to rounding), fused in one call (m25x_cgs2) for up to 8 systems; the small Hessenberg
 columns come to the host once per iteration for the conjugated Givens rotations, the convergence test and the
-stall cut-off, as the reference computes them.
+stall cut-off, as the reference computes them. The next iteration's device work (preconditioner, product,
+orthogonalisation) does not depend on those, so it is queued before the host waits for the current column
+(through pinned buffers and an event), unless the residual's recent rate predicts convergence at this iteration;
+an iteration queued past the end is discarded.
 """
 import numpy as np
 import torch
 
 from . import device as dv
+
+
+_PINNED = {}
+
+
+def _pinned(shape):
+    """Two pinned host buffers of complex128 (shape), kept per shape."""
+    b = _PINNED.get(shape)
+    if b is None:
+        b = _PINNED[shape] = [torch.empty(shape, dtype=torch.complex128, pin_memory=True) for _ in range(2)]
+    return b
 
 
 def _cnorm(X):
@@ -48,16 +62,36 @@ def gmres_batched(A, B, M=None, x0=None, rtol=1e-6, restart=30, maxiter=10, stal
         g[0] = beta
         V[0] = R / torch.from_numpy(beta).to(dev)[None, :]
         live, last, stalled = np.ones(ka, bool), np.full(ka, restart - 1), False
+        rn = np.zeros((restart + 1, ka))                             # residual estimates, for the prediction
+        rn[0] = beta
         hbuf = torch.empty((restart + 1, ka), dtype=torch.complex128, device=dev)
+        # classical Gram-Schmidt twice (as stable as modified Gram-Schmidt): one fused device call for up to 8
+        # systems and 63 basis vectors (then pipelined one iteration ahead), batched tensor products beyond
+        fused = ka <= 8 and restart < 64
+        if fused:
+            pin, evs, queued = _pinned((restart + 1, ka)), [torch.cuda.Event(), torch.cuda.Event()], -1
+
+            def enqueue(i):
+                Zs[i] = M(V[i])
+                dv.cgs2(V, i + 1, A(Zs[i]).contiguous(), hbuf)
+                pin[i % 2][:i + 2].copy_(hbuf[:i + 2], non_blocking=True)
+                evs[i % 2].record()
         for j in range(restart):
-            Zs[j] = M(V[j])
-            Wv = A(Zs[j]).contiguous()
-            # classical Gram-Schmidt twice (as stable as modified Gram-Schmidt): one fused device call for up to 8
-            # systems and 63 basis vectors, batched tensor products beyond
-            if ka <= 8 and j + 1 < 64:
-                dv.cgs2(V, j + 1, Wv, hbuf)
-                hc = hbuf[:j + 2].cpu().numpy()
+            if fused:
+                if queued < j:
+                    enqueue(j)
+                    queued = j
+                # queue the next iteration unless this one is predicted to converge (the last step's ratio)
+                ratio = rn[j] / np.maximum(rn[j - 1], 1e-300) if j else np.ones(ka)
+                soon = (rn[j] * ratio <= 2 * rtol * bn[act])[live].all()
+                if j + 1 < restart and not soon:
+                    enqueue(j + 1)
+                    queued = j + 1
+                evs[j % 2].synchronize()
+                hc = pin[j % 2][:j + 2].numpy().copy()
             else:
+                Zs[j] = M(V[j])
+                Wv = A(Zs[j]).contiguous()
                 Vj = V[:j + 1]
                 h1 = (Vj.conj() * Wv[None]).sum(1)
                 Wv = Wv - (Vj * h1[:, None, :]).sum(0)
@@ -75,6 +109,7 @@ def gmres_batched(A, B, M=None, x0=None, rtol=1e-6, restart=30, maxiter=10, stal
             cs[j], sn[j] = H[j, j].conj() / den, H[j + 1, j].conj() / den
             H[j, j], H[j + 1, j] = cs[j] * H[j, j] + sn[j] * H[j + 1, j], 0
             g[j + 1], g[j] = -np.conj(sn[j]) * g[j], cs[j] * g[j]
+            rn[j + 1] = np.abs(g[j + 1])
             iters[act[live]] += 1
             conv = live & (np.abs(g[j + 1]) <= rtol * bn[act])
             last[conv] = j

@@ -39,3 +39,44 @@ def test_cgs2_matches_numpy(gpu, n, k, nv):
         for c in range(k):
             assert abs(np.linalg.norm(vn[:, c]) - 1) < 1e-12
             assert np.abs(V[:nv, :, c].conj() @ vn[:, c]).max() < 1e-12
+
+
+def _seeded_system():
+    """The reference's seeded complex system (tests/test_solver.py:375-398), as the oracle's GMRES pin uses it."""
+    rng = np.random.default_rng(1)
+    n = 200
+    ev = (1 + rng.random(n)) * np.exp(1j * rng.uniform(-1.2, 1.2, n))
+    Qm = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))[0]
+    A = Qm @ np.diag(ev) @ Qm.conj().T + 0.2 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+    B = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    good = np.linalg.inv(A + 1e-3 * rng.standard_normal((n, n)))
+    bad = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    return A, B, good, bad
+
+
+@pytest.mark.parametrize("case", ["plain", "restarts", "good", "bad_stall", "bad_full"])
+def test_device_gmres_matches_oracle(gpu, case):
+    """mom25d_amd.linalg.gmres_batched (device basis, fused CGS2, the next iteration queued ahead) against the
+    oracle's restatement: the same iteration counts per column and the same solution to the tolerance."""
+    from oracle.gmres import gmres_batched as ref_gmres
+    from mom25d_amd.linalg import gmres_batched
+    A, B, good, bad = _seeded_system()
+    kw = dict(plain=dict(rtol=1e-10, restart=60, maxiter=5), restarts=dict(rtol=1e-10, restart=7, maxiter=20),
+              good=dict(M=good, rtol=1e-8, restart=30, maxiter=1, stall=(4, 0.1)),
+              bad_stall=dict(M=bad, rtol=1e-8, restart=30, maxiter=3, stall=(4, 0.1)),
+              bad_full=dict(M=bad, rtol=1e-8, restart=30, maxiter=1))[case]
+    Mh = kw.pop('M', None)
+    Xr, itr = ref_gmres(lambda V: A @ V, B, M=(lambda R: Mh @ R) if Mh is not None else None, **kw)
+    dev = gpu.device()
+    Ad, Bd = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
+    Md = torch.from_numpy(Mh).to(dev) if Mh is not None else None
+    Xd, itd = gmres_batched(lambda V: Ad @ V, Bd, M=(lambda R: Md @ R) if Md is not None else None, **kw)
+    Xd = Xd.cpu().numpy()
+    assert np.isfinite(Xd).all()
+    assert max(abs(a - b) for a, b in zip(itd, itr)) <= 1, (itd, itr)   # (rounding may move a threshold crossing)
+    if case in ("plain", "restarts", "good"):
+        res = np.linalg.norm(A @ Xd - B, axis=0) / np.linalg.norm(B, axis=0)
+        assert res.max() < 10 * kw['rtol']
+        assert np.abs(Xd - Xr).max() <= 100 * kw['rtol'] * np.abs(Xr).max()
+    else:
+        assert itd == itr
