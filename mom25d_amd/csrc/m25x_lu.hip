@@ -226,12 +226,16 @@ __global__ void k_laswp(T *A, long lda, long n, long j0, int jb, const int *ipiv
     }
 }
 
-// U12 = L11^-1 A12: rows [j0, j0 + jb), columns [c0, n); 32 columns a workgroup, the block in LDS
+// U12 = L11^-1 A12: rows [j0, j0 + jb), columns [c0, n); TC columns a workgroup, the block in LDS. JB: the largest
+// panel width; with JB = TW (the tournament's panels) L11 is staged in LDS too (column k of L11 as row k of Lt, so a
+// step's reads are consecutive), else read from memory at each step.
 constexpr int TC = 16;
-template <typename T>
+template <typename T, int JB>
 __global__ __launch_bounds__(256) void k_trsm(T *A, long lda, long n, long j0, int jb, long c0)
 {
-    __shared__ T X[NB][TC + 1];
+    constexpr bool STAGE = JB <= TW;
+    __shared__ T X[JB][TC + 1];
+    __shared__ T Lt[STAGE ? TW : 1][TW];
     const long cb = c0 + (long)blockIdx.x * TC;
     const int nc = (int)std::min((long)TC, n - cb);
     const int tid = threadIdx.x;
@@ -239,13 +243,21 @@ __global__ __launch_bounds__(256) void k_trsm(T *A, long lda, long n, long j0, i
         const int c = idx / jb, r = idx % jb;
         if (c < nc) X[r][c] = A[(cb + c) * lda + j0 + r];
     }
-    __syncthreads();
     const T *L = A + j0 * lda + j0;          // L11[i][k] = L[k * lda + i]
+    if constexpr (STAGE)
+        for (int idx = tid; idx < jb * jb; idx += 256) {
+            const int k = idx / jb, i = idx % jb;
+            Lt[k][i] = L[(long)k * lda + i];
+        }
+    __syncthreads();
     for (int k = 0; k < jb - 1; ++k) {
         const int nr = jb - k - 1;
         for (int idx = tid; idx < nr * TC; idx += 256) {
             const int c = idx % TC, i = k + 1 + idx / TC;
-            if (c < nc) X[i][c] = tsub(X[i][c], tmul(L[(long)k * lda + i], X[k][c]));
+            T lik;
+            if constexpr (STAGE) lik = Lt[k][i];
+            else lik = L[(long)k * lda + i];
+            if (c < nc) X[i][c] = tsub(X[i][c], tmul(lik, X[k][c]));
         }
         __syncthreads();
     }
@@ -416,13 +428,14 @@ __global__ __launch_bounds__(256) void k_l21(T *A, long lda, long r0, long m, lo
         auto solve = [&](auto jc) __attribute__((always_inline)) {
             constexpr int j = decltype(jc)::value;
             if (32 * h + j < jb) {
-                T v = x.get(j);
+                T v = x.get(j), v1 = CX<T>::make(0, 0);      // two partial sums: half the dependent chain
                 auto sub = [&](auto kc) __attribute__((always_inline)) {
                     constexpr int k = decltype(kc)::value;
-                    v = tsub(v, tmul(x.get(k), Us[32 * h + k][32 * h + j]));
+                    if constexpr (k % 2 == 0) v = tsub(v, tmul(x.get(k), Us[32 * h + k][32 * h + j]));
+                    else v1 = tadd(v1, tmul(x.get(k), Us[32 * h + k][32 * h + j]));
                 };
                 Unroll<0, j>::run(sub);
-                x.set(j, tmul(v, rd[32 * h + j]));
+                x.set(j, tmul(tsub(v, v1), rd[32 * h + j]));
             }
         };
         Unroll<0, TH>::run(solve);
@@ -821,7 +834,7 @@ int getrf_t(T *A, long n, long lda, int *ipiv, int *info, void *work, size_t wby
         }
         const long c0 = j0 + jb;
         if (c0 < n) {
-            hipLaunchKernelGGL(k_trsm<T>, dim3(blocks(n - c0, TC)), dim3(256), 0, s, A, lda, n, j0, jb, c0);
+            hipLaunchKernelGGL((k_trsm<T, NB>), dim3(blocks(n - c0, TC)), dim3(256), 0, s, A, lda, n, j0, jb, c0);
             if (int rc = check_launch("getrf/trsm")) return rc;
             const long M = n - c0;
             dim3 grid(blocks(M, GM), blocks(M, GN));
@@ -870,7 +883,7 @@ int getrf_tp(T *A, long n, long lda, int *ipiv, int *perm, int *info, void *work
         const long c0 = j0 + jb;
         if (c0 < n) {   // U12 = L11^-1 A12, A22 -= L21 U12
             T *A12 = A + c0 * lda + j0;
-            hipLaunchKernelGGL(k_trsm<T>, dim3(blocks(n - c0, TC)), dim3(256), 0, s, A, lda, n, j0, jb, c0);
+            hipLaunchKernelGGL((k_trsm<T, TW>), dim3(blocks(n - c0, TC)), dim3(256), 0, s, A, lda, n, j0, jb, c0);
             const long M = n - c0;
             hipLaunchKernelGGL((k_gemm<T, true, double2>), dim3(blocks(M, GM), blocks(M, GN)), dim3(256), 0, s,
                                A + c0 * lda + c0, lda, (const T *)(A + j0 * lda + c0), lda, (const T *)A12, lda, M, M,
