@@ -159,8 +159,10 @@ size_t m25x_getrs_work_bytes(int dtype, long n, long ldb);
 /* Solves with the factor of m25x_getrf: trans = 1: Z X = B (A^T X = B, lu_solve(..., trans=1)), trans = 0:
  * A X = B. B: n x nrhs row-major (leading dimension ldb >= nrhs, 1 <= nrhs <= 8), the factor's dtype, overwritten
  * with X. perm: m25x_getrf's (or m25x_ipiv_perm's, or NULL: computed here). Diagonal blocks by substitution, as
- * LAPACK's: the accuracy of a factor with small pivots is kept. Replaces the backend's solve (src/mom25d/gpu.py:76-80;
- * src/mom25d/gpu_vk.py:650-678) and zgetrs / cgetrs (src/mom25d/fast.py:356-358). */
+ * LAPACK's: the accuracy of a factor with small pivots is kept. Every wait between the solve's workgroups is
+ * bounded: the int at work + work_bytes - 256 is nonzero after a call in which one ran out (X is then not the
+ * solution), zero otherwise. Replaces the backend's solve (src/mom25d/gpu.py:76-80; src/mom25d/gpu_vk.py:650-678)
+ * and zgetrs / cgetrs (src/mom25d/fast.py:356-358). */
 int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B, long ldb,
                int nrhs, int trans, void *work, size_t work_bytes, void *stream);
 

@@ -895,7 +895,8 @@ int getrf_tp(T *A, long n, long lda, int *ipiv, int *perm, int *info, void *work
 }
 
 // Solve workspace: y of the two solves (W1, W2), written as self-marking values over UNSET words; the gathered
-// right-hand sides (W0); 1 / U_ii; the permutation and the abort word.
+// right-hand sides (W0); 1 / U_ii; the permutation, the abort word (at work_bytes - 256: callers read it there)
+// and 63 spare ints.
 template <typename T> size_t getrs_work(long n, long)
 {
     return sizeof(T) * (3 * (size_t)n * MAXRHS + (size_t)n) + sizeof(int) * ((size_t)n + 64);

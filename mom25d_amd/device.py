@@ -315,8 +315,30 @@ def getrs(F, B, trans=1):
         work = torch.empty(wb, dtype=torch.uint8, device=B.device)
         L.call('m25x_getrs', dt, ptr(F.LU), n, F.LU.stride(0), ptr(F.ipiv), ptr(F.perm), ptr(X), kc, kc,
                int(trans), ptr(work), wb, stream())
+        _solve_fault(B.device).bitwise_or_(work[wb - 256:wb - 252].view(torch.int32))   # the call's abort word
         out[:, c0:c0 + kc] = X.to(c128)
     return out[:, 0] if one else out
+
+
+_FAULTS = {}
+
+
+def _solve_fault(dev):
+    f = _FAULTS.get(dev)
+    if f is None:
+        f = _FAULTS[dev] = torch.zeros(1, dtype=torch.int32, device=dev)
+    return f
+
+
+def solve_faults(dev=None):
+    """Whether a getrs since the last call ran out of a wait bound between its workgroups (its result is then not
+    the solution); clears the record. Reading it synchronises with the stream."""
+    f = _FAULTS.get(torch.device(dev) if dev is not None else device())
+    if f is None:
+        return False
+    bad = bool(f.item())
+    f.zero_()
+    return bad
 
 
 def trsv_timing(on):

@@ -149,8 +149,12 @@ class FastOperator:
             self._check_info(F)
             with sim.log.timer('LU solve'):
                 X = dsc[:, None] * dv.getrs(F, B, trans=1)
-            return X.cpu().numpy()
-        return self._solve_kept(f, Z, dsc, B, t_fresh).cpu().numpy()
+        else:
+            X = self._solve_kept(f, Z, dsc, B, t_fresh)
+        if dv.solve_faults(dev):
+            raise RuntimeError(f"a triangular solve at {f / 1e9:.4g} GHz ran out of a wait bound between its "
+                               "workgroups: its result is not the solution")
+        return X.cpu().numpy()
 
     def _check_info(self, F):
         info = int(F.info.item())

@@ -89,7 +89,10 @@ class HipBackend(_Backend):
         (src/mom25d/gpu.py:76-80)."""
         R = np.asarray(R)
         Rd = self.torch.from_numpy(np.ascontiguousarray(R.astype(complex))).to(self.dv.device())
-        return self.solve_device(Rd).cpu().numpy()
+        X = self.solve_device(Rd).cpu().numpy()
+        if self.dv.solve_faults():
+            raise RuntimeError("solve: a triangular solve ran out of a wait bound: its result is not the solution")
+        return X
 
     def solve_device(self, Rd, trans=1):
         """Z^-1 R (trans 1) or Z^-T R (trans 0) for a device tensor R; complex128 device tensor out."""

@@ -153,8 +153,8 @@ def test_getrf_c64_tournament(gpu, n):
 @pytest.mark.parametrize('dt', [np.complex64, np.complex128])
 def test_getrs_right_hand_side_counts(gpu, dt):
     """The solves for 1 to 8 right-hand sides at sizes with and without a partial last 64-row block and with long
-    block chains (the chain workgroup, its helpers and its look-ahead couplings all in play), both transposes,
-    against NumPy's solve; the caller's B is left as it was."""
+    chains of 64-row blocks (one workgroup each, every hand-off between them in play), both transposes, against
+    NumPy's solve; the caller's B is left as it was, and no wait between workgroups ran out."""
     dv = gpu
     tol = 1e-4 if dt == np.complex64 else 1e-10
     for n in (1, 63, 64, 200, 777, 3001):
@@ -171,3 +171,4 @@ def test_getrs_right_hand_side_counts(gpu, dt):
                 assert torch.equal(Rd, R0), (n, k, trans)
                 x = np.linalg.solve(Zc if trans else Zc.T, R)
                 assert np.abs(X - x).max() <= tol * np.abs(x).max(), (n, k, trans)
+    assert not dv.solve_faults()
