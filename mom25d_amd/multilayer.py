@@ -170,18 +170,37 @@ class LayerStack:
             cache[key] = out
         return out
 
+    def _layer_factors(self, kr, w, xp):
+        """Per layer j: kz_j and the transfer factors tn_j, sc_j, which the two modes share (kept on a device kr
+        tensor with the networks)."""
+        cache = getattr(kr, '_m25x_networks', None) if xp is torch else None
+        key = ('layers', float(w)) + self.key()
+        if cache is not None and key in cache:
+            return cache[key]
+        k0, N = w / C0, self.N
+        kz, tn, sc = [None] * (N + 1), [None] * (N + 1), [None] * (N + 1)
+        for j in range(1, N + 1):
+            kz[j] = -1j * xp.sqrt(kr * kr - k0 * k0 * self.eps(j))
+            th = kz[j] * self.layers[j - 1].h
+            E = xp.exp(-2j * th)
+            tn[j], sc[j] = -1j * (1 - E) / (1 + E), 2 * xp.exp(-1j * th) / (1 + E)
+        if xp is torch:
+            if cache is None:
+                cache = kr._m25x_networks = {}
+            cache[key] = (kz, tn, sc)
+        return kz, tn, sc
+
     def _network_eval(self, kr, w, mode, xp):
         k0, N = w / C0, self.N
 
         def y0(e):
             kz = -1j * xp.sqrt(kr * kr - k0 * k0 * e)
             return (kz / (w * MU0) if mode == 'h' else w * EPS0 * e / kz), kz
-        Y0, tn, sc = [None] * (N + 1), [None] * (N + 1), [None] * (N + 1)
+        kzs, tn, sc = self._layer_factors(kr, w, xp)
+        Y0 = [None] * (N + 1)
         for j in range(1, N + 1):
-            Y0[j], kz = y0(self.eps(j))
-            th = kz * self.layers[j - 1].h
-            E = xp.exp(-2j * th)
-            tn[j], sc[j] = -1j * (1 - E) / (1 + E), 2 * xp.exp(-1j * th) / (1 + E)
+            e, kz = self.eps(j), kzs[j]
+            Y0[j] = kz / (w * MU0) if mode == 'h' else w * EPS0 * e / kz
         Yd = [None] * (N + 1)
         if self.bottom == 'pec':
             Yd[1], kfirst = -1j * Y0[1] / tn[1], 2
