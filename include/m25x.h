@@ -141,11 +141,13 @@ size_t m25x_getrf_work_bytes(int dtype, long n);
 
 /* LU of the column-major n x n matrix A (leading dimension lda; the row-major Z seen as Z^T), in place: L unit
  * lower, U upper, ipiv 0-based LAPACK-style interchanges (int32, device), the layout of
- * scipy.linalg.lu_factor(Z.T). complex128 (trailing updates on FP64 MFMA) or complex64 (FP32 MFMA, FP32
- * accumulation). pivot: M25X_PIVOT_PARTIAL or M25X_PIVOT_TOURNAMENT. perm (device int[n], may be NULL): the
- * interchanges as one permutation, m25x_ipiv_perm's. info (device int, may be NULL): 0, or 1 + the first zero
- * pivot. Replaces the GPU backend's factor (src/mom25d/gpu.py:69-74; src/mom25d/gpu_vk.py:515-633) and LAPACK
- * zgetrf/cgetrf (src/mom25d/fast.py:318-320). */
+ * scipy.linalg.lu_factor(Z.T). complex128: trailing updates on the FP64 matrix cores. complex64: with partial
+ * pivoting on the FP32 matrix cores (FP32 accumulation); with tournament pivoting on the FP64 matrix cores (one
+ * rounding to FP32 an entry). pivot: M25X_PIVOT_PARTIAL or M25X_PIVOT_TOURNAMENT. perm (device int[n], may be
+ * NULL): the interchanges as one permutation, m25x_ipiv_perm's. info (device int, may be NULL): 0, 1 + the first
+ * zero pivot, or -1 when a bounded wait between the workgroups of a partial-pivoting panel ran out (the factor is
+ * then not valid; the tournament has no such wait). Replaces the GPU backend's factor (src/mom25d/gpu.py:69-74;
+ * src/mom25d/gpu_vk.py:515-633) and LAPACK zgetrf/cgetrf (src/mom25d/fast.py:318-320). */
 int m25x_getrf(int dtype, int pivot, void *A, long n, long lda, int *ipiv, int *perm, int *info, void *work,
                size_t work_bytes, void *stream);
 
@@ -160,20 +162,37 @@ size_t m25x_getrs_work_bytes(int dtype, long n, long ldb);
  * A X = B. B: n x nrhs row-major (leading dimension ldb >= nrhs, 1 <= nrhs <= 8), the factor's dtype, overwritten
  * with X. perm: m25x_getrf's (or m25x_ipiv_perm's, or NULL: computed here). Diagonal blocks by substitution, as
  * LAPACK's: the accuracy of a factor with small pivots is kept. Every wait between the solve's workgroups is
- * bounded: the int at work + work_bytes - 256 is nonzero after a call in which one ran out (X is then not the
- * solution), zero otherwise. Replaces the backend's solve (src/mom25d/gpu.py:76-80; src/mom25d/gpu_vk.py:650-678)
+ * bounded: the int at work + (work_bytes - 256, rounded down to a multiple of 4), with the work_bytes passed, is
+ * nonzero after a call in which one ran out (X is then not the solution), zero otherwise; it is zeroed once per
+ * call, so it covers every right-hand side of the call. Replaces the backend's solve (src/mom25d/gpu.py:76-80; src/mom25d/gpu_vk.py:650-678)
  * and zgetrs / cgetrs (src/mom25d/fast.py:356-358). */
 int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B, long ldb,
                int nrhs, int trans, void *work, size_t work_bytes, void *stream);
 
-/* Launch timing of the triangular solves, for benchmarks (not thread-safe). m25x_trsv_timing(1) starts a record
- * (clearing the last): from then on each triangular-solve kernel that m25x_getrs launches is bracketed by two HIP
- * events on its stream; m25x_trsv_timing(0) stops recording. m25x_trsv_times waits for the last recorded launch
- * and writes, for up to cap launches, its duration (ms) and the bytes it reads and writes by algorithm (its
- * triangle of the factor, the right-hand sides once each way); it returns the number of launches recorded, or a
- * negative error code. No counterpart in the reference: its solve is a LAPACK / cuSOLVER call. */
-int m25x_trsv_timing(int on);
-long m25x_trsv_times(float *ms, double *bytes, long cap);
+/* ---- launch timing (benchmarks) ------------------------------------------------------------------------------
+ * No counterpart in the reference (its fill and LU are CPU code, LAPACK or a vendor library). m25x_timing(mask)
+ * opens a record (clearing the last) in which every launch of a kernel whose bit (1 << M25X_K_*) is in mask is
+ * bracketed by two HIP events on its stream; m25x_timing(0) closes it. m25x_timing_records waits for the recorded
+ * launches and writes, for up to cap of them in launch order, the kernel, the element type (M25X_C128 / M25X_C64),
+ * the duration (ms), and the bytes and flops the launch moves and does by algorithm (0 where the kernel has no
+ * such count); it returns the number recorded, or a negative error code. Thread-safe; one record per process. */
+#define M25X_K_FILL_FAR 0       /* 16 B (8 B) per Z entry written */
+#define M25X_K_NEAR_MOMENTS 1   /* per near pair: 7 complex moments out */
+#define M25X_K_NEAR_SCATTER 2   /* per near entry: read-modify-write of Z, its contributions read */
+#define M25X_K_TSLU 3           /* a 64-column panel's tournament: the panel read once; flops: its GEPPs */
+#define M25X_K_APPLY_MOVES 4    /* the panel's row moves: 2 x element per moved element */
+#define M25X_K_L21 5            /* L21 = A21 U11^-1: 8 m 64^2 / 2 flops */
+#define M25X_K_TRSM 6           /* U12 = L11^-1 A12: 8 n 64^2 / 2 flops */
+#define M25X_K_GEMM 7           /* trailing update: 8 M N K flops; bytes: C read and written, A and B read */
+#define M25X_K_TRSV 8           /* a triangular solve: its triangle of the factor and the right-hand sides */
+#define M25X_K_ZGEMM_NK 9       /* Y = Z X: 16 N^2 B of Z read */
+#define M25X_K_CGS2 10          /* one Arnoldi step's orthogonalisation */
+#define M25X_K_PANEL 11         /* a partial-pivoting panel */
+#define M25X_K_LASWP 12         /* its row interchanges */
+int m25x_timing(unsigned kernels);
+long m25x_timing_records(int *kernel, int *dtype, float *ms, double *bytes, double *flops, long cap);
+/* The kernel's name ("" for an unknown id). */
+const char *m25x_kernel_name(int kernel);
 
 /* ---- refinement products ------------------------------------------------------------------------------- */
 

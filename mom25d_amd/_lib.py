@@ -50,8 +50,9 @@ _SIGS = {
     'm25x_ipiv_perm': ([P, L, P, P], I),
     'm25x_getrs_work_bytes': ([I, L, L], S),
     'm25x_getrs': ([I, P, L, L, P, P, P, L, I, I, P, S, P], I),
-    'm25x_trsv_timing': ([I], I),
-    'm25x_trsv_times': ([P, P, L], L),
+    'm25x_timing': ([C.c_uint], I),
+    'm25x_timing_records': ([P, P, P, P, P, L], L),
+    'm25x_kernel_name': ([I], C.c_char_p),
     'm25x_zgemm_nk': ([P, L, L, P, I, P, P], I),
     'm25x_cgs2_work_bytes': ([], S),
     'm25x_cgs2': ([P, L, I, I, P, P, P, P, S, P], I),

@@ -79,6 +79,7 @@ extern "C" int m25x_zgemm_nk(const double *Z, long n, long ld, const double *X, 
     hipStream_t s = (hipStream_t)stream;
     const double2 *Zc = (const double2 *)Z, *Xc = (const double2 *)X;
     double2 *Yc = (double2 *)Y;
+    TimedLaunch tl(s, M25X_K_ZGEMM_NK, M25X_C128, 16.0 * n * (double)n + 32.0 * n * k, 8.0 * n * (double)n * k);
     switch (k) {
 #define M25X_NK(K)                                                                                            \
     case K:                                                                                                   \
@@ -241,6 +242,7 @@ extern "C" int m25x_cgs2(const double *V, long n, int k, int nv, double *w, doub
     double2 *x = (double2 *)w, *H = (double2 *)hcol;
     double2 *P = (double2 *)work, *h = P + (size_t)GS_B * GS_MAXV * 8;
     double *Q = (double *)(h + GS_MAXV * 8);
+    TimedLaunch tl(s, M25X_K_CGS2, M25X_C128, 16.0 * n * k * (2.0 * nv + 3), 16.0 * n * k * nv);
     hipLaunchKernelGGL(k_gs_dots<false>, dim3(GS_B), dim3(GS_T), 0, s, Vc, n, k, nv, x, (const double2 *)nullptr, P);
     hipLaunchKernelGGL(k_gs_sum, dim3(1), dim3(GS_MAXV * 8), 0, s, (const double2 *)P, k, nv, h, H, 1);
     hipLaunchKernelGGL(k_gs_dots<true>, dim3(GS_B), dim3(GS_T), 0, s, Vc, n, k, nv, x, (const double2 *)h, P);

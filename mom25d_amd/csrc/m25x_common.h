@@ -15,6 +15,21 @@ int fail(int code, const char *fmt, ...);
 int check_launch(const char *what);
 int num_cus();
 
+// ---- launch timing (benchmarks) ---------------------------------------------------------------------------
+// While a record is open (m25x_timing with the kernel's bit in its mask), a TimedLaunch around a launch brackets
+// it with two HIP events on its stream and notes the launch's algorithmic bytes and flops.
+class TimedLaunch {
+  public:
+    TimedLaunch(hipStream_t s, int kernel, int dtype, double bytes, double flops);
+    ~TimedLaunch();
+    TimedLaunch(const TimedLaunch &) = delete;
+    TimedLaunch &operator=(const TimedLaunch &) = delete;
+
+  private:
+    hipStream_t s_;
+    long slot_;
+};
+
 #define M25X_HIP(call)                                                                                        \
     do {                                                                                                      \
         hipError_t e_ = (call);                                                                               \

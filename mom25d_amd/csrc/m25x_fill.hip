@@ -579,6 +579,7 @@ int m25x_fill_far(void *Z, int zdtype, long ld, const m25x_mesh *g, const m25x_t
     const int nt = (g->Ne + FT - 1) / FT;
     dim3 grid(nt, nt);
     hipStream_t s = (hipStream_t)stream;
+    TimedLaunch tl(s, M25X_K_FILL_FAR, zdtype, (zdtype == M25X_C128 ? 16.0 : 8.0) * g->Ne * (double)g->Ne, 0.0);
     if (zdtype == M25X_C128)
         hipLaunchKernelGGL(k_fill_far<double2>, grid, dim3(256), 0, s, (double2 *)Z, ld, dmesh(g), dtab(tb), w);
     else if (zdtype == M25X_C64)
@@ -604,6 +605,7 @@ int m25x_near_moments(const m25x_mesh *g, const m25x_tables *tb, const m25x_quad
     if (!g || !tb || !q || n < 0 || !M) return fail(M25X_EINVAL, "near_moments: bad arguments");
     if (n == 0) return M25X_OK;
     DQuad d{q->nodes, q->tris, q->P3, q->W3, q->P6, q->W6};
+    TimedLaunch tl((hipStream_t)stream, M25X_K_NEAR_MOMENTS, M25X_C128, 7.0 * 16 * n, 0.0);
     hipLaunchKernelGGL(k_near_moments, dim3(blocks(n, 256)), dim3(256), 0, (hipStream_t)stream, dmesh(g), dtab(tb),
                        d, P, Q, n, stat, stat_ready, (double2 *)M, (double2 *)qcorr);
     return check_launch("near_moments");
@@ -616,6 +618,7 @@ int m25x_near_scatter(void *Z, int zdtype, long ld, const m25x_mesh *g, const in
     if (!Z || !g || nentry < 0) return fail(M25X_EINVAL, "near_scatter: bad arguments");
     if (nentry == 0) return M25X_OK;
     hipStream_t s = (hipStream_t)stream;
+    TimedLaunch tl(s, M25X_K_NEAR_SCATTER, zdtype, (zdtype == M25X_C128 ? 32.0 : 16.0) * nentry, 0.0);
     if (zdtype == M25X_C128)
         hipLaunchKernelGGL(k_near_scatter<double2>, dim3(blocks(nentry, 256)), dim3(256), 0, s, (double2 *)Z, ld,
                            dmesh(g), P, Q, (const double2 *)M, nentry, erow, ecol, estart, contrib, w);

@@ -27,6 +27,8 @@ using namespace m25x::lu;
 
 namespace {
 
+template <typename T> constexpr int dtype_of() { return sizeof(T) == 16 ? M25X_C128 : M25X_C64; }
+
 // ---- grid barrier (all workgroups of a cooperative launch co-resident) -------------------------------------
 struct Bar {
     unsigned count, gen;
@@ -35,11 +37,13 @@ struct Bar {
 
 // Sense by generation; release / acquire at agent scope orders the slot writes and reads across the XCDs.
 // A bounded wait: past the limit the barrier sets abort and every later barrier returns at once, so the
-// launch always drains (the host then reports the failure).
-__device__ inline void grid_sync(Bar *b, unsigned nblocks, unsigned &gen)
+// launch always drains. `aborted` becomes true in a workgroup that timed out or skipped a barrier for it; the
+// panel kernel then writes -1 into the factor's info word, which the caller reads (the factor is not valid).
+__device__ inline void grid_sync(Bar *b, unsigned nblocks, unsigned &gen, bool &aborted)
 {
     __threadfence();
     __syncthreads();
+    __shared__ int s_abort;
     if (threadIdx.x == 0) {
         if (!__hip_atomic_load(&b->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
             const unsigned arrived = __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
@@ -57,9 +61,11 @@ __device__ inline void grid_sync(Bar *b, unsigned nblocks, unsigned &gen)
                 }
             }
         }
+        s_abort = __hip_atomic_load(&b->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         gen += 1;
     }
     __syncthreads();
+    aborted = aborted || s_abort != 0;
 }
 
 __global__ void k_zero_ints(int *p, int n)
@@ -124,6 +130,7 @@ __global__ __launch_bounds__(PT) void k_panel(T *A, long lda, long n, long j0, i
     const long r0 = j0 + (long)g * R;
     const int rn = (int)std::max(0L, std::min((long)R, n - r0));
     unsigned gen = 0;
+    bool aborted = false;
     for (int idx = tid; idx < rn * jb; idx += PT) {
         const int c = idx / rn, r = idx % rn;
         S[c * R + r] = A[(j0 + c) * lda + r0 + r];
@@ -146,7 +153,7 @@ __global__ __launch_bounds__(PT) void k_panel(T *A, long lda, long n, long j0, i
             for (int c = tid; c < jb; c += PT) diag[(size_t)buf * NB + c] = S[c * R + (first - r0)];
     };
     publish(0, 0);
-    grid_sync(bar, G, gen);
+    grid_sync(bar, G, gen, aborted);
     for (int k = 0; k < jb; ++k) {
         const int buf = k & 1;
         const long dr = j0 + k;
@@ -198,13 +205,14 @@ __global__ __launch_bounds__(PT) void k_panel(T *A, long lda, long n, long j0, i
         __syncthreads();
         if (k + 1 < jb) {
             publish(k + 1, buf ^ 1);
-            grid_sync(bar, G, gen);
+            grid_sync(bar, G, gen, aborted);
         }
     }
     for (int idx = tid; idx < rn * jb; idx += PT) {
         const int c = idx / rn, r = idx % rn;
         A[(j0 + c) * lda + r0 + r] = S[c * R + r];
     }
+    if (aborted && tid == 0 && info) __hip_atomic_exchange(info, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // the panel's interchanges on columns [0, j0) and [j0 + jb, n): one thread a column, in order
@@ -826,20 +834,32 @@ int getrf_t(T *A, long n, long lda, int *ipiv, int *info, void *work, size_t wby
         const size_t smem = (size_t)R * jb * sizeof(T);
         void *args[] = {(void *)&A, (void *)&lda, (void *)&n, (void *)&j0, (void *)&jb, (void *)&R, (void *)&ipiv,
                         (void *)&info, (void *)&slots, (void *)&diag, (void *)&bar};
-        M25X_HIP(hipLaunchCooperativeKernel((const void *)k_panel<T>, dim3(G), dim3(PT), args, (unsigned)smem, s));
+        const int dt = dtype_of<T>();
+        const double es = sizeof(T);
+        {
+            TimedLaunch tl(s, M25X_K_PANEL, dt, 2.0 * es * m * jb, 0.0);
+            M25X_HIP(hipLaunchCooperativeKernel((const void *)k_panel<T>, dim3(G), dim3(PT), args, (unsigned)smem, s));
+        }
         const long nsw = n - jb;
         if (nsw > 0) {
+            TimedLaunch tl(s, M25X_K_LASWP, dt, 2.0 * es * 2 * jb * nsw, 0.0);
             hipLaunchKernelGGL(k_laswp<T>, dim3(blocks(nsw, 256)), dim3(256), 0, s, A, lda, n, j0, jb, ipiv);
             if (int rc = check_launch("getrf/laswp")) return rc;
         }
         const long c0 = j0 + jb;
         if (c0 < n) {
-            hipLaunchKernelGGL((k_trsm<T, NB>), dim3(blocks(n - c0, TC)), dim3(256), 0, s, A, lda, n, j0, jb, c0);
+            {
+                TimedLaunch tl(s, M25X_K_TRSM, dt, 2.0 * es * (n - c0) * jb, 4.0 * (n - c0) * jb * jb);
+                hipLaunchKernelGGL((k_trsm<T, NB>), dim3(blocks(n - c0, TC)), dim3(256), 0, s, A, lda, n, j0, jb, c0);
+            }
             if (int rc = check_launch("getrf/trsm")) return rc;
             const long M = n - c0;
             dim3 grid(blocks(M, GM), blocks(M, GN));
-            hipLaunchKernelGGL((k_gemm<T, true>), grid, dim3(256), 0, s, A + c0 * lda + c0, lda, A + j0 * lda + c0, lda,
-                               A + c0 * lda + j0, lda, M, M, (long)jb);
+            {
+                TimedLaunch tl(s, M25X_K_GEMM, dt, es * (2.0 * M * M + 2.0 * M * jb), 8.0 * M * M * jb);
+                hipLaunchKernelGGL((k_gemm<T, true>), grid, dim3(256), 0, s, A + c0 * lda + c0, lda, A + j0 * lda + c0,
+                                   lda, A + c0 * lda + j0, lda, M, M, (long)jb);
+            }
             if (int rc = check_launch("getrf/gemm")) return rc;
         }
     }
@@ -873,18 +893,33 @@ int getrf_tp(T *A, long n, long lda, int *ipiv, int *perm, int *info, void *work
         const int jb = (int)std::min((long)TW, n - j0);
         const long m = n - j0;
         const int P = (int)tour_leaves(m, RL);
-        if (int rc = launch_tslu((const T *)A, lda, n, j0, jb, P, nodes, arrive, top, ipiv, pairs, info, s)) return rc;
-        hipLaunchKernelGGL(k_apply_moves<T>, dim3(blocks(n, 8) + (perm ? 1 : 0)), dim3(256), 0, s, A, lda, 0L, n,
-                           (const int *)pairs, j0, jb, (const T *)top, perm);
+        const int dt = dtype_of<T>();
+        const double es = sizeof(T);
+        {
+            TimedLaunch tl(s, M25X_K_TSLU, dt, es * m * jb, 0.0);
+            if (int rc = launch_tslu((const T *)A, lda, n, j0, jb, P, nodes, arrive, top, ipiv, pairs, info, s))
+                return rc;
+        }
+        {
+            TimedLaunch tl(s, M25X_K_APPLY_MOVES, dt, 2.0 * es * 2 * jb * n, 0.0);
+            hipLaunchKernelGGL(k_apply_moves<T>, dim3(blocks(n, 8) + (perm ? 1 : 0)), dim3(256), 0, s, A, lda, 0L, n,
+                               (const int *)pairs, j0, jb, (const T *)top, perm);
+        }
         if (int rc = check_launch("getrf/panel")) return rc;
-        if (m > jb)     // L21 = A21 U11^-1
+        if (m > jb) {   // L21 = A21 U11^-1
+            TimedLaunch tl(s, M25X_K_L21, dt, 2.0 * es * (m - jb) * jb, 4.0 * (m - jb) * jb * jb);
             hipLaunchKernelGGL(k_l21<T>, dim3(blocks(m - jb, 256)), dim3(256), 0, s, A, lda, j0 + jb, m - jb, j0, jb,
                                (const T *)top);
+        }
         const long c0 = j0 + jb;
         if (c0 < n) {   // U12 = L11^-1 A12, A22 -= L21 U12
             T *A12 = A + c0 * lda + j0;
-            hipLaunchKernelGGL((k_trsm<T, TW>), dim3(blocks(n - c0, TC)), dim3(256), 0, s, A, lda, n, j0, jb, c0);
+            {
+                TimedLaunch tl(s, M25X_K_TRSM, dt, 2.0 * es * (n - c0) * jb, 4.0 * (n - c0) * jb * jb);
+                hipLaunchKernelGGL((k_trsm<T, TW>), dim3(blocks(n - c0, TC)), dim3(256), 0, s, A, lda, n, j0, jb, c0);
+            }
             const long M = n - c0;
+            TimedLaunch tl(s, M25X_K_GEMM, dt, es * (2.0 * M * M + 2.0 * M * jb), 8.0 * M * M * jb);
             hipLaunchKernelGGL((k_gemm<T, true, double2>), dim3(blocks(M, GM), blocks(M, GN)), dim3(256), 0, s,
                                A + c0 * lda + c0, lda, (const T *)(A + j0 * lda + c0), lda, (const T *)A12, lda, M, M,
                                (long)jb);
@@ -895,80 +930,37 @@ int getrf_tp(T *A, long n, long lda, int *ipiv, int *perm, int *info, void *work
 }
 
 // Solve workspace: y of the two solves (W1, W2), written as self-marking values over UNSET words; the gathered
-// right-hand sides (W0); 1 / U_ii; the permutation, the abort word (at work_bytes - 256: callers read it there)
+// right-hand sides (W0); 1 / U_ii; the permutation, the abort word (at the passed work_bytes - 256, rounded down
+// to an int: callers read it there)
 // and 63 spare ints.
 template <typename T> size_t getrs_work(long n, long)
 {
     return sizeof(T) * (3 * (size_t)n * MAXRHS + (size_t)n) + sizeof(int) * ((size_t)n + 64);
 }
 
-// ---- launch timing of the triangular solves (benchmarks; one host thread) -----------------------------------
-// While on, every k_trsv launch is bracketed by two HIP events recorded on its stream, with the bytes it reads by
-// algorithm (its triangle of the factor, and the right-hand sides read and written once).
-struct TrsvTiming {
-    bool on = false;
-    std::vector<hipEvent_t> ev;
-    std::vector<double> bytes;
-    size_t used = 0;
-};
-TrsvTiming g_tt;
-
-int tt_mark(hipStream_t s)
-{
-    if (!g_tt.on) return M25X_OK;
-    if (g_tt.used == g_tt.ev.size()) {
-        hipEvent_t e;
-        M25X_HIP(hipEventCreate(&e));
-        g_tt.ev.push_back(e);
-    }
-    M25X_HIP(hipEventRecord(g_tt.ev[g_tt.used++], s));
-    return M25X_OK;
-}
-
 template <typename T>
 int trsv_launch(hipStream_t s, int nb, const T *LU, long lda, long n, const T *rd, const T *src, long lds, T *dst,
                 int nrhs, int kind, int *abort)
 {
-    if (int rc = tt_mark(s)) return rc;
+    const double tri = kind_fwd(kind) ? 0.5 * n * (n - 1) : 0.5 * n * (n + 1);
+    TimedLaunch tl(s, M25X_K_TRSV, dtype_of<T>(), (tri + 2.0 * n * nrhs) * sizeof(T), 8.0 * tri * nrhs);
     if (nrhs <= 4)
         hipLaunchKernelGGL((k_trsv<T, 4>), dim3(nb), dim3(TB), 0, s, LU, lda, n, rd, src, lds, dst, nrhs, kind, abort);
     else
         hipLaunchKernelGGL((k_trsv<T, TrsvMaxRhs<T>::NR>), dim3(nb), dim3(TB), 0, s, LU, lda, n, rd, src, lds, dst,
                            nrhs, kind, abort);
-    if (int rc = check_launch("getrs/trsv")) return rc;
-    if (g_tt.on) {
-        const double tri = kind_fwd(kind) ? 0.5 * n * (n - 1) : 0.5 * n * (n + 1);
-        g_tt.bytes.push_back((tri + 2.0 * n * nrhs) * sizeof(T));
-    }
-    return tt_mark(s);
+    return check_launch("getrs/trsv");
 }
 
+// One solve of at most TrsvMaxRhs<T>::NR right-hand sides; the abort word is the caller's (zeroed once per call).
 template <typename T>
-int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, T *B, long ldb, int nrhs, int trans,
-            void *work, size_t wbytes, hipStream_t s)
+int getrs_part(const T *LU, long n, long lda, const int *perm_in, T *B, long ldb, int nrhs, int trans, T *W1, T *W2,
+               T *W0, const T *rd, int *abort, hipStream_t s)
 {
-    if (nrhs < 1 || nrhs > MAXRHS) return fail(M25X_EINVAL, "getrs: 1 to %d right-hand sides, not %d", MAXRHS, nrhs);
-    if (nrhs > TrsvMaxRhs<T>::NR) {                      // in two halves (the solve's LDS holds fewer)
-        const int h = TrsvMaxRhs<T>::NR;
-        if (int rc = getrs_t<T>(LU, n, lda, ipiv, perm_in, B, ldb, h, trans, work, wbytes, s)) return rc;
-        return getrs_t<T>(LU, n, lda, ipiv, perm_in, B + h, ldb, nrhs - h, trans, work, wbytes, s);
-    }
-    if (ldb < nrhs) return fail(M25X_EINVAL, "getrs: ldb < nrhs");
-    if (!work || wbytes < getrs_work<T>(n, ldb))
-        return fail(M25X_EINVAL, "getrs: workspace of %zu bytes, %zu needed", wbytes, getrs_work<T>(n, ldb));
     const int nb = (int)((n + SB - 1) / SB);
     const size_t wn = (size_t)n * MAXRHS;
-    T *W1 = (T *)work, *W2 = W1 + wn, *W0 = W2 + wn, *rd = W0 + wn;
-    int *perm = (int *)(rd + n);
-    int *abort = perm + n;
     const long words = (long)(2 * wn * sizeof(T) / 8);
     hipLaunchKernelGGL(k_fill_unset, dim3(blocks(words, 256)), dim3(256), 0, s, (unsigned long long *)W1, words);
-    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, abort, 1);
-    hipLaunchKernelGGL(k_rdiag<T>, dim3(blocks(n, 256)), dim3(256), 0, s, LU, lda, n, rd);
-    if (!perm_in) {
-        hipLaunchKernelGGL(k_perm, dim3(1), dim3(256), 0, s, ipiv, n, perm);
-        perm_in = perm;
-    }
     if (int rc = check_launch("getrs")) return rc;
     const unsigned pg = blocks(n * nrhs, 256);
     int rc;
@@ -984,6 +976,44 @@ int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, 
                            (const int *)nullptr, 0);
     }
     return check_launch("getrs");
+}
+
+// The solve with the buffers at buf (getrs_work(n) - 256 bytes: W1, W2, W0, 1 / U_ii, the permutation) and the
+// abort word given (not reset here): in parts of at most TrsvMaxRhs<T>::NR right-hand sides.
+template <typename T>
+int getrs_noreset(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, T *B, long ldb, int nrhs,
+                  int trans, char *buf, int *abort, hipStream_t s)
+{
+    const size_t wn = (size_t)n * MAXRHS;
+    T *W1 = (T *)buf, *W2 = W1 + wn, *W0 = W2 + wn, *rd = W0 + wn;
+    int *perm = (int *)(rd + n);
+    hipLaunchKernelGGL(k_rdiag<T>, dim3(blocks(n, 256)), dim3(256), 0, s, LU, lda, n, rd);
+    if (!perm_in) {
+        hipLaunchKernelGGL(k_perm, dim3(1), dim3(256), 0, s, ipiv, n, perm);
+        perm_in = perm;
+    }
+    if (int rc = check_launch("getrs")) return rc;
+    for (int c0 = 0; c0 < nrhs; c0 += TrsvMaxRhs<T>::NR) {
+        const int kc = std::min(TrsvMaxRhs<T>::NR, nrhs - c0);
+        if (int rc = getrs_part<T>(LU, n, lda, perm_in, B + c0, ldb, kc, trans, W1, W2, W0, rd, abort, s)) return rc;
+    }
+    return M25X_OK;
+}
+
+// The solve: buffers carved from `work` (W1, W2, W0: n x MAXRHS each; 1 / U_ii; the permutation), the abort word
+// at work + wbytes - 256 (rounded down to an int), zeroed once here and raised by any part whose wait ran out.
+// More right-hand sides than the solve kernel holds are solved in parts that share the word.
+template <typename T>
+int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, T *B, long ldb, int nrhs, int trans,
+            void *work, size_t wbytes, hipStream_t s)
+{
+    if (nrhs < 1 || nrhs > MAXRHS) return fail(M25X_EINVAL, "getrs: 1 to %d right-hand sides, not %d", MAXRHS, nrhs);
+    if (ldb < nrhs) return fail(M25X_EINVAL, "getrs: ldb < nrhs");
+    if (!work || wbytes < getrs_work<T>(n, ldb))
+        return fail(M25X_EINVAL, "getrs: workspace of %zu bytes, %zu needed", wbytes, getrs_work<T>(n, ldb));
+    int *abort = (int *)((char *)work + ((wbytes - 256) & ~(size_t)3));
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, abort, 1);
+    return getrs_noreset<T>(LU, n, lda, ipiv, perm_in, B, ldb, nrhs, trans, (char *)work, abort, s);
 }
 
 }  // namespace
@@ -1032,33 +1062,6 @@ int m25x_ipiv_perm(const int *ipiv, long n, int *perm, void *stream)
     if (n == 0) return M25X_OK;
     hipLaunchKernelGGL(k_perm, dim3(1), dim3(256), 0, (hipStream_t)stream, ipiv, n, perm);
     return check_launch("ipiv_perm");
-}
-
-int m25x_trsv_timing(int on)
-{
-    if (on) {
-        g_tt.used = 0;
-        g_tt.bytes.clear();
-    }
-    g_tt.on = on != 0;
-    return M25X_OK;
-}
-
-long m25x_trsv_times(float *ms, double *bytes, long cap)
-{
-    const long n = (long)g_tt.bytes.size();
-    if (n && 2 * (size_t)n <= g_tt.used) {
-        hipError_t e = hipEventSynchronize(g_tt.ev[2 * n - 1]);
-        if (e != hipSuccess) return fail(M25X_EHIP, "trsv_times: %s", hipGetErrorString(e));
-    }
-    for (long i = 0; i < n && i < cap; ++i) {
-        if (ms) {
-            hipError_t e = hipEventElapsedTime(ms + i, g_tt.ev[2 * i], g_tt.ev[2 * i + 1]);
-            if (e != hipSuccess) return fail(M25X_EHIP, "trsv_times: %s", hipGetErrorString(e));
-        }
-        if (bytes) bytes[i] = g_tt.bytes[i];
-    }
-    return n;
 }
 
 int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B, long ldb,

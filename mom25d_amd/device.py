@@ -341,23 +341,37 @@ def solve_faults(dev=None):
     return bad
 
 
-def trsv_timing(on):
-    """Start (clearing the last record) or stop timing the triangular-solve launches of getrs: HIP events on the
-    stream each launch runs on."""
-    L.call('m25x_trsv_timing', int(bool(on)))
+KERNELS = ('fill_far', 'near_moments', 'near_scatter', 'tslu', 'apply_moves', 'l21', 'trsm', 'gemm', 'trsv',
+           'zgemm_nk', 'cgs2', 'panel', 'laswp')
 
 
-def trsv_times():
-    """(ms, bytes) per triangular-solve launch recorded since trsv_timing(True): durations from the events, and the
-    bytes each launch reads and writes by algorithm (its triangle of the factor, the right-hand sides)."""
+def timing(kernels):
+    """Open a launch-timing record for the named kernels (KERNELS; 'all' for every one), clearing the last, or close
+    it (kernels empty): each recorded launch is bracketed by two HIP events on the stream it runs on."""
+    if kernels == 'all':
+        kernels = KERNELS
+    mask = 0
+    for k in kernels:
+        mask |= 1 << KERNELS.index(k)
+    L.call('m25x_timing', mask)
+
+
+def timing_records():
+    """The launches recorded since timing(...): a dict of arrays kernel (names), dtype ('c128' / 'c64'), ms, and
+    the bytes and flops each launch moves and does by algorithm (0 where the kernel has no such count)."""
     lib = L.lib()
-    n = lib.m25x_trsv_times(None, None, 0)
+    n = lib.m25x_timing_records(None, None, None, None, None, 0)
     if n < 0:
         raise RuntimeError(lib.m25x_last_error().decode())
-    ms, by = np.zeros(n, np.float32), np.zeros(n, np.float64)
+    k, d = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    ms, by, fl = np.zeros(n, np.float32), np.zeros(n, np.float64), np.zeros(n, np.float64)
     if n:
-        lib.m25x_trsv_times(ms.ctypes.data_as(C.c_void_p), by.ctypes.data_as(C.c_void_p), n)
-    return ms.astype(np.float64), by
+        p = lambda a: a.ctypes.data_as(C.c_void_p)
+        got = lib.m25x_timing_records(p(k), p(d), p(ms), p(by), p(fl), n)
+        if got < 0:
+            raise RuntimeError(lib.m25x_last_error().decode())
+    return dict(kernel=np.array([KERNELS[i] for i in k], dtype=object),
+                dtype=np.where(d == L.C128, 'c128', 'c64'), ms=ms.astype(np.float64), bytes=by, flops=fl)
 
 
 def zgemm_nk(Z, X):

@@ -6,7 +6,8 @@ Per frequency, everything on the device:
   fill     Z (N x N complex128): far field, near-field correction (pairs and their analytic moments resident
            across frequencies), via rows / columns and block, lumped elements
   solve    'double': Z equilibrated (D Z D, d = 1/sqrt|Z_ii|), complex128 LU, solves
-           'mixed' : the complex64 copy of the equilibrated Z factored (FP32 matrix cores), GMRES refinement
+           'mixed' : the complex64 copy of the equilibrated Z factored (trailing updates on the FP64 matrix
+                     cores, one rounding to FP32 an entry), GMRES refinement
                      against the FP64 Z (restart 30, stall (4, 0.1) for a fresh factor, converged at 100 x 1e-6);
                      a kept factor of a nearby frequency tried first (its reach learnt); where the complex64
                      factor does not converge, complex128 LU, and double precision up to MIXED_RETRY x f
@@ -134,8 +135,7 @@ class FastOperator:
     def solve_direct(self, f, E, precision='double'):
         """Currents (host N x k complex128) for excitation columns E at f (fast.py:704-850)."""
         sim = self.sim
-        if precision == 'mixed' and f <= MIXED_RETRY * self._mixed_failed:
-            precision = 'double'
+        near_failed = precision == 'mixed' and f <= MIXED_RETRY * self._mixed_failed
         t_fresh = time.time()
         Z = self.fill(f)
         dev = Z.device
@@ -150,7 +150,7 @@ class FastOperator:
             with sim.log.timer('LU solve'):
                 X = dsc[:, None] * dv.getrs(F, B, trans=1)
         else:
-            X = self._solve_kept(f, Z, dsc, B, t_fresh)
+            X = self._solve_kept(f, Z, dsc, B, t_fresh, fresh='double' if near_failed else 'mixed')
         if dv.solve_faults(dev):
             raise RuntimeError(f"a triangular solve at {f / 1e9:.4g} GHz ran out of a wait bound between its "
                                "workgroups: its result is not the solution")
@@ -158,14 +158,19 @@ class FastOperator:
 
     def _check_info(self, F):
         info = int(F.info.item())
+        if info < 0:
+            raise RuntimeError("the LU's panel ran out of a wait bound between its workgroups: the factor is not valid")
         if info:
             self.sim.log.note(f"WARNING the LU met an exactly zero pivot at {info - 1}: the matrix is singular")
 
-    def _solve_kept(self, f, Z, dsc, B, t_fresh):
-        """Mixed precision with complex64 factors kept on the GPU (fast.py:996-1084)."""
+    def _solve_kept(self, f, Z, dsc, B, t_fresh, fresh='mixed'):
+        """Mixed precision with factors kept on the GPU (fast.py:996-1084): a kept factor of a nearby frequency
+        first; else a fresh complex64 one (fresh='mixed') or, within MIXED_RETRY of a frequency where complex64 did not
+        converge, a fresh complex128 one (fresh='double'). A complex128 factor is kept like a complex64 one, so the
+        points after a failed one are preconditioned by it instead of each taking its own complex128 LU."""
         sim = self.sim
         n = Z.shape[0]
-        Z64 = torch.empty((n, n), dtype=dv.c64, device=Z.device)
+        Z64 = torch.empty((n, n), dtype=dv.c64, device=Z.device) if fresh == 'mixed' else None
         dv.scale_c64(Z, dsc, Z64)                               # Z <- D Z D (kept, FP64), and its complex64 copy
         A = lambda V: dv.zgemm_nk(Z, V)
         k = self._reusable(f)
@@ -184,6 +189,8 @@ class FastOperator:
                 return dsc[:, None] * Y
             sim.log.note(f"the LU of {f0 / 1e9:.4g} GHz did not precondition {f / 1e9:.4g} GHz in {cap} iterations "
                          f"(residual {rel:.1e}) - a new one")
+        if fresh == 'double':
+            return self._fresh_double(f, Z, dsc, B, t_fresh)
         with sim.log.timer('LU factor (GPU)', f"N={n}, complex64"):
             F = dv.getrf(Z64)
         self._check_info(F)
@@ -202,6 +209,17 @@ class FastOperator:
         sim.log.note(f"WARNING mixed precision did not converge at {f / 1e9:.4g} GHz (residual {rel:.1e}) - solving "
                      f"in double precision up to {MIXED_RETRY * f / 1e9:.4g} GHz")
         self._mixed_failed = max(self._mixed_failed, f)
-        with sim.log.timer('LU factor', f"N={n}, complex128, GPU"):
-            Fd = dv.getrf(Z)                                     # Z is equilibrated already
-        return dsc[:, None] * dv.getrs(Fd, B, trans=1)
+        return self._fresh_double(f, Z, dsc, B, t_fresh)
+
+    def _fresh_double(self, f, Z, dsc, B, t_fresh):
+        """The complex128 LU of the equilibrated Z and its solve; the factor is kept for the nearby points."""
+        sim = self.sim
+        with sim.log.timer('LU factor', f"N={Z.shape[0]}, complex128, GPU"):
+            Fd = dv.getrf(Z)                                     # in place: the factor owns Z's storage from here
+        self._check_info(Fd)
+        X = dsc[:, None] * dv.getrs(Fd, B, trans=1)
+        if sim.reuse != 'off':
+            self._kept.append((f, Fd, dsc, time.time() - t_fresh))
+            while len(self._kept) > REUSE_KEEP:
+                self._drop_kept()
+        return X

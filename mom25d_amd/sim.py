@@ -20,10 +20,9 @@ from .mesh import contains, find_node, tensor_mesh
 from .multilayer import RunLog, device_tables, green_tables
 from . import device as dv
 
-MIXED_MIN_N = 3500
-GPU_MIN_N = 3000
-HYBRID_MAX_N = 25000
-BLR_MIN_N = 20000
+GPU_MIN_N = 3000          # mixed precision (complex64 LU + FP64 refinement) above this many unknowns
+REUSE_KEEP = 4            # kept factors a solver holds at most (mom25d_amd.fast.REUSE_KEEP)
+WORK_BYTES = 0.4e9        # the LU's and the solves' workspaces, the tables, the mesh and the near plan, bounded
 
 
 def _zfun(Z):
@@ -31,17 +30,21 @@ def _zfun(Z):
 
 
 def rlc(R=0.0, L=0.0, C=None, parallel=False):
-    """Impedance function of a series (default) or parallel R-L-C element (sim.py:47-54)."""
-    def Z(f):
-        w = 2 * np.pi * f
-        if not parallel:
-            return R + 1j * w * L + (1 / (1j * w * C) if C else 0)
-        return 1 / ((1 / R if R else 0) + (1 / (1j * w * L) if L else 0) + (1j * w * C if C else 0))
-    return Z
+    """The impedance function f -> Z of an R-L-C element, series (default) or parallel (the reference's rlc,
+    src/mom25d/sim.py:47-54)."""
+    def series(f):
+        jw = 2j * np.pi * f
+        return R + jw * L + (1 / (jw * C) if C else 0)
+
+    def shunt(f):
+        jw = 2j * np.pi * f
+        y = (1 / R if R else 0) + (1 / (jw * L) if L else 0) + (jw * C if C else 0)
+        return 1 / y
+    return shunt if parallel else series
 
 
 def available_memory():
-    """Bytes of host memory available (the reference's emsdev.memory.available)."""
+    """Bytes of host memory available now (MemAvailable), the reference's emsdev.memory.available."""
     try:
         with open('/proc/meminfo') as fh:
             for line in fh:
@@ -52,47 +55,45 @@ def available_memory():
     return 16e9
 
 
-def memory_plan(n, has_core=True, avail=None, gpu=None, precision='auto', solver='auto', n_lu=None, has_blr=False):
-    """How an n-unknown solve runs and its memory (sim.py:85-151): dict(solver, precision, path, host, gpu, fits).
-    gpu: None, (name, free bytes) or (name, free, the LU's bytes, fits). On this build the direct solver's 'hybrid'
-    path keeps every matrix on the GPU (the double Z, its complex64 copy and the factors), so host needs are
-    small; the plan's figures keep the reference's meaning for comparison."""
-    avail = available_memory() if avail is None else avail
-    name, free = gpu[:2] if gpu else (None, 0.0)
-    if gpu and len(gpu) >= 4:
-        gpu_lu, gpu_fits = float(gpu[2]), bool(gpu[3])
-    else:
-        gpu_lu = (1 if name in ('cupy', 'emulate', 'hip') else 2) * 8.0 * n * n + 0.4e9
-        gpu_fits = name is not None and gpu_lu <= 0.9 * free
-    if precision == 'auto':
-        m = n_lu or n
-        precision = 'mixed' if m > (GPU_MIN_N if gpu_fits else MIXED_MIN_N) or 18.0 * n * n + 0.9e9 > 0.5 * avail \
-            else 'double'
-    if precision == 'mixed' and not has_core:
-        precision = 'double'
-    hybrid = precision == 'mixed' and gpu_fits and 24.0 * n * n + 0.9e9 <= 0.8 * avail and (n_lu or n) <= HYBRID_MAX_N
+def device_bytes(n, precision, keep=REUSE_KEEP):
+    """Device bytes one n-unknown solve holds on the GPU in this build, where every matrix stays there: 'double' -
+    the complex128 Z factored in place (16 n^2); 'mixed' - the FP64 Z the refinement multiplies by (16 n^2), the
+    complex64 factor (8 n^2) and up to `keep` kept factors of nearby frequencies (8 n^2 each, complex128 ones 16);
+    plus WORK_BYTES."""
+    nn = float(n) * n
     if precision == 'double':
-        path, host, dev = 'double', 18.0 * n * n + 0.9e9, 0.0
-    elif hybrid:
-        path, host, dev = 'hybrid', 24.0 * n * n + 0.9e9, gpu_lu
-    elif gpu_fits:
-        path, host, dev = 'gpu-lu', 10.0 * n * n + 0.9e9, gpu_lu
-    else:
-        path, host, dev = 'cpu-mixed', 10.0 * n * n + 0.9e9, 0.0
-    fits = host <= 0.8 * avail
-    if solver == 'auto':
-        solver = 'dense' if n <= 3500 and fits and not has_core else ('direct' if fits else ('blr' if has_core else 'aca'))
-        if solver == 'direct' and has_blr and path in ('gpu-lu', 'gpu-stream', 'cpu-mixed') and n >= BLR_MIN_N:
-            solver = 'blr'
-    if solver == 'blr' and not has_blr:
-        solver = 'aca'
-    if solver == 'dense':
-        path, host, dev = 'dense', 18.0 * n * n + 0.9e9, 0.0
-    elif solver == 'blr':
-        path, host, dev = solver, 3.0 * n * n + 2e4 * n + 0.9e9, 0.0
-    elif solver == 'aca':
-        path, host, dev = solver, 2e4 * n + 0.9e9, 0.0
-    return dict(solver=solver, precision=precision, path=path, host=host, gpu=dev, fits=fits)
+        return 16.0 * nn + WORK_BYTES
+    return 24.0 * nn + 8.0 * nn * keep + WORK_BYTES
+
+
+def memory_plan(n, has_core=True, avail=None, gpu=None, precision='auto', solver='auto', n_lu=None, has_blr=False,
+                keep=REUSE_KEEP):
+    """How an n-unknown solve runs and what it holds: dict(solver, precision, path, host, gpu, fits), the
+    reference's plan (src/mom25d/sim.py:85-151) for this build, whose direct solver keeps everything on the GPU.
+
+    gpu: None, (name, free bytes) or (name, free bytes, the LU's bytes, fits) as the reference's gpu_info gives.
+    precision 'auto': 'mixed' above GPU_MIN_N unknowns of the LU (n_lu, default n), else 'double'; a requested
+    precision is kept. path: 'hybrid' (complex64 LU and FP64 refinement, both on the GPU) or 'double' (complex128
+    LU on the GPU). gpu (the result): the device bytes of device_bytes(); host: the bytes the host holds (the
+    mesh, geometry and currents, O(n)); fits: whether the device bytes fit 0.9 x the free device memory. The
+    solver is 'direct' (dense LU) whatever is asked: the compressed solvers (BLR, ACA) and the host paths
+    (cpu-mixed, gpu-stream) are not part of this build, and a request that does not fit is reported as such
+    (fits False) for the caller to queue or refuse."""
+    if n < 0:
+        raise ValueError("memory_plan: n must be >= 0")
+    free = float(gpu[1]) if gpu else 0.0
+    m = n_lu or n
+    if precision == 'auto':
+        precision = 'mixed' if m > GPU_MIN_N else 'double'
+    if precision not in ('mixed', 'double'):
+        raise ValueError(f"memory_plan: precision {precision!r}")
+    dev = device_bytes(n, precision, keep)
+    if gpu and len(gpu) >= 4 and precision == 'mixed':
+        dev = max(dev, float(gpu[2]) + 16.0 * n * n)        # the backend's own estimate of its factor, and Z
+    fits = gpu is not None and dev <= 0.9 * free
+    path = 'hybrid' if precision == 'mixed' else 'double'
+    host = 2e4 * n + 64 * 16.0 * n + 0.2e9
+    return dict(solver='direct', precision=precision, path=path, host=host, gpu=dev, fits=fits)
 
 
 class Via:
@@ -251,9 +252,7 @@ class Simulation:
         self.rho_max = 1.05 * np.hypot(*(r.nodes.max(0) - r.nodes.min(0))) + 1e-4
         self._edge_sets()
         n = self.Ne + self.Nv
-        plan = memory_plan(n, True, available_memory(), ('hip', self._gpu_free(), 24.0 * n * n + 0.4e9,
-                                                           24.0 * n * n + 0.4e9 <= 0.9 * self._gpu_free()),
-                           precision, 'direct')
+        plan = memory_plan(n, gpu=('hip', self._gpu_free()), precision=precision)
         self.plan, self.precision, self.solver = plan, plan['precision'], 'direct'
         self.log.note(f"solver: direct on the GPU (N = {n}, {self.precision} precision)")
         from .fast import FastOperator

@@ -1,0 +1,41 @@
+"""The device Green's tables and via kernels against the reference's own values (tests/golden/tables_c1_c2.npz,
+written by tests/golden/make_tables.py from the reference's multilayer.py / via.py with its compiled core): the
+microstrip stack (C1) and the seeded 4-layer stack (C2) at 1, 3.7 and 8 GHz, every interface pair: A_rest, Q_rest
+and D to 1e-10 of each table's largest value, R + homog to 1e-10 of each value, the extraction scalars to 1e-12."""
+import numpy as np
+import pytest
+
+from tests.test_oracle_golden import cases, gold, rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize('name', ['c1', 'c2'])
+def test_device_tables_match_reference(gpu, name):
+    from mom25d_amd.multilayer import Layer, LayerStack, green_tables
+    from mom25d_amd.via import ViaKernels
+    G = gold()
+    idx = G['idx']
+    for nm, lay, ifs, dmax, rho_max, spans in cases(G):
+        if nm != name:
+            continue
+        st = LayerStack([Layer(h, er, td) for h, er, td in lay])
+        for fi, f in enumerate(G['freqs']):
+            tabs = green_tables(f, st, ifs, rho_max, dmax)
+            for (a, b), t in tabs.items():
+                key = f'{name}/f{fi}/t{a}{b}'
+                assert rel(t.A_rest.cpu().numpy()[idx], G[key + '/A']) < 1e-10, key
+                assert rel(t.Q_rest.cpu().numpy()[idx], G[key + '/Q']) < 1e-10, key
+                sc = G[key + '/scal']
+                mine = np.array([t.d, t.dimg, t.cA, t.cq, t.cAi, t.cqi], complex)
+                assert np.all(np.abs(mine - sc[:6]) <= 1e-12 * np.maximum(np.abs(sc[:6]), 1e-300)), key
+            vk = ViaKernels(f, st, spans, ifs, rho_max)
+            for (k, span), tab in vk.D.items():
+                key = f'{name}/f{fi}/D{k}_{span[0]}{span[1]}'
+                assert rel(tab.v.cpu().numpy()[idx], G[key]) < 1e-10, key
+            for sm in sorted(set(spans)):
+                for sn in sorted(set(spans)):
+                    key = f'{name}/f{fi}/R{sm[0]}{sm[1]}_{sn[0]}{sn[1]}'
+                    ref = G[key]
+                    mine = vk.R(sm, sn, G['rho_r']) + vk.homog(sm, sn, G['rho_r'])
+                    assert np.all(np.abs(mine - ref) <= 1e-10 * np.abs(ref)), key

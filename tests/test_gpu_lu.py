@@ -172,3 +172,30 @@ def test_getrs_right_hand_side_counts(gpu, dt):
                 x = np.linalg.solve(Zc if trans else Zc.T, R)
                 assert np.abs(X - x).max() <= tol * np.abs(x).max(), (n, k, trans)
     assert not dv.solve_faults()
+
+
+def test_getrs_abort_word_covers_split_calls(gpu):
+    """A complex128 solve of 8 right-hand sides runs as two parts of 4 inside one m25x_getrs call; the call's abort
+    word sits at work + work_bytes - 256 for the work_bytes passed (here twice the minimum, the buffer pre-filled
+    with 0xff bytes), is zeroed once by the call and still reads zero after both parts; the solution is right."""
+    import ctypes as C
+    from mom25d_amd import _lib as L
+    dv = gpu
+    n, k = 777, 8
+    Z = seeded_matrix(n)
+    F = _factor(dv, Z, 'tournament')
+    rng = np.random.default_rng(5)
+    R = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
+    X = torch.from_numpy(R.copy()).to(dv.device())
+    wb = 2 * L.lib().m25x_getrs_work_bytes(L.C128, n, k) + 12
+    work = torch.full((wb,), 255, dtype=torch.uint8, device=dv.device())
+    L.call('m25x_getrs', L.C128, dv.ptr(F.LU), n, n, dv.ptr(F.ipiv), dv.ptr(F.perm), dv.ptr(X), k, k, 1,
+           dv.ptr(work), wb, dv.stream())
+    torch.cuda.synchronize()
+    at = (wb - 256) & ~3
+    assert int(work[at:at + 4].view(torch.int32).item()) == 0
+    x = np.linalg.solve(Z, R)
+    assert np.abs(X.cpu().numpy() - x).max() <= 1e-10 * np.abs(x).max()
+    with pytest.raises(ValueError):                        # a workspace below the minimum is refused
+        L.call('m25x_getrs', L.C128, dv.ptr(F.LU), n, n, dv.ptr(F.ipiv), dv.ptr(F.perm), dv.ptr(X), k, k, 1,
+               dv.ptr(work), 1024, dv.stream())
