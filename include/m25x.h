@@ -169,6 +169,19 @@ size_t m25x_getrs_work_bytes(int dtype, long n, long ldb);
 int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B, long ldb,
                int nrhs, int trans, void *work, size_t work_bytes, void *stream);
 
+/* The solve of m25x_getrs (the same kernels, the same results) with its workspace in a device array of this
+ * library instead of a caller's buffer: it allocates nothing (no hipMalloc, no allocating library call), for
+ * callers that cannot allocate - code under HIP graph capture, serving loops and harnesses that forbid it. No
+ * counterpart in the reference. The array holds M25X_GETRS_SCRATCH_BYTES on each device the library runs on,
+ * until the process ends; n is limited to what fits (m25x_getrs_work_bytes(dtype, n, ldb) - 256 bytes: about
+ * n <= 41 000 complex128, 82 000 complex64 at ldb = 8), larger n is rejected with M25X_EINVAL. status (device int):
+ * zeroed by the call, nonzero afterwards when a wait between the solve's workgroups ran out (X is then not the
+ * solution). Every call on one device shares that device's array: calls that overlap on one device (two streams,
+ * two threads) race and give wrong results; serialise them. */
+#define M25X_GETRS_SCRATCH_BYTES (16u << 20)
+int m25x_getrs_noalloc(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B,
+                       long ldb, int nrhs, int trans, int *status, void *stream);
+
 /* ---- launch timing (benchmarks) ------------------------------------------------------------------------------
  * No counterpart in the reference (its fill and LU are CPU code, LAPACK or a vendor library). m25x_timing(mask)
  * opens a record (clearing the last) in which every launch of a kernel whose bit (1 << M25X_K_*) is in mask is

@@ -9,6 +9,7 @@ LIB_PATH = os.path.join(_HERE, 'libm25x.so')
 C128, C64 = 0, 1
 EINVAL, EHIP, ENOMEM, ESINGULAR = -1, -2, -3, -4
 PIVOT_PARTIAL, PIVOT_TOURNAMENT = 0, 1
+GETRS_SCRATCH_BYTES = 16 << 20
 
 
 class Tables(C.Structure):
@@ -50,6 +51,7 @@ _SIGS = {
     'm25x_ipiv_perm': ([P, L, P, P], I),
     'm25x_getrs_work_bytes': ([I, L, L], S),
     'm25x_getrs': ([I, P, L, L, P, P, P, L, I, I, P, S, P], I),
+    'm25x_getrs_noalloc': ([I, P, L, L, P, P, P, L, I, I, P, P], I),
     'm25x_timing': ([C.c_uint], I),
     'm25x_timing_records': ([P, P, P, P, P, L], L),
     'm25x_kernel_name': ([I], C.c_char_p),

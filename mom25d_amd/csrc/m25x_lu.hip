@@ -1016,9 +1016,71 @@ int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, 
     return getrs_noreset<T>(LU, n, lda, ipiv, perm_in, B, ldb, nrhs, trans, (char *)work, abort, s);
 }
 
+// ---- the solve without allocation ------------------------------------------------------------------------
+// The workspace of m25x_getrs_noalloc: one array per device, in the code object of this file, held until the process
+// ends. Host code never passes the array's name to a kernel or a copy (on the host it is only a stand-in): the
+// device address comes from hipGetSymbolAddress, once per device, and is kept in scratch_dev.
+static __device__ __align__(256) unsigned char scratch[M25X_GETRS_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void *scratch_dev[MAX_DEVICES];
+
+// The current device's address of `scratch` (looked up at the first call on that device).
+int scratch_of_current(char **out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES)
+        return fail(M25X_EHIP, "getrs_noalloc: no current device");
+    std::lock_guard<std::mutex> g(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(M25X_EHIP, "getrs_noalloc: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        scratch_dev[dev] = p;
+    }
+    *out = (char *)scratch_dev[dev];
+    return M25X_OK;
+}
+
+template <typename T>
+int getrs_noalloc_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, T *B, long ldb, int nrhs,
+                    int trans, int *status, hipStream_t s)
+{
+    if (nrhs < 1 || nrhs > MAXRHS) return fail(M25X_EINVAL, "getrs_noalloc: 1 to %d right-hand sides, not %d", MAXRHS, nrhs);
+    if (ldb < nrhs) return fail(M25X_EINVAL, "getrs_noalloc: ldb < nrhs");
+    // the buffers getrs_noreset carves (W1, W2, W0, 1 / U_ii, the permutation): getrs_work less its 256-byte tail
+    const size_t need = getrs_work<T>(n, ldb) - 256;
+    if (need > (size_t)M25X_GETRS_SCRATCH_BYTES)
+        return fail(M25X_EINVAL, "getrs_noalloc: n = %ld needs %zu bytes of workspace, more than the %u held per device",
+                    n, need, (unsigned)M25X_GETRS_SCRATCH_BYTES);
+    char *buf = nullptr;
+    if (int rc = scratch_of_current(&buf)) return rc;
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, status, 1);
+    if (int rc = check_launch("getrs_noalloc")) return rc;
+    return getrs_noreset<T>(LU, n, lda, ipiv, perm_in, B, ldb, nrhs, trans, buf, status, s);
+}
+
 }  // namespace
 
 extern "C" {
+
+int m25x_getrs_noalloc(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B,
+                       long ldb, int nrhs, int trans, int *status, void *stream)
+{
+    if (!LU || !ipiv || !B || !status || n < 0 || lda < n) return fail(M25X_EINVAL, "getrs_noalloc: bad arguments");
+    if (dtype != M25X_C128 && dtype != M25X_C64) return fail(M25X_EINVAL, "getrs_noalloc: dtype %d", dtype);
+    hipStream_t s = (hipStream_t)stream;
+    if (n == 0) {
+        hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, status, 1);
+        return check_launch("getrs_noalloc");
+    }
+    if (dtype == M25X_C128)
+        return getrs_noalloc_t<double2>((const double2 *)LU, n, lda, ipiv, perm, (double2 *)B, ldb, nrhs, trans,
+                                        status, s);
+    return getrs_noalloc_t<float2>((const float2 *)LU, n, lda, ipiv, perm, (float2 *)B, ldb, nrhs, trans, status, s);
+}
 
 size_t m25x_getrf_work_bytes(int dtype, long n)
 {

@@ -320,6 +320,21 @@ def getrs(F, B, trans=1):
     return out[:, 0] if one else out
 
 
+def getrs_noalloc(F, B, trans=1, status=None):
+    """getrs through m25x_getrs_noalloc: the same solve with the library's own per-device workspace (nothing is
+    allocated by the library; the caller's B is solved in place, 1 <= k <= 8 columns, the factor's dtype, n within
+    the workspace's limit). status: a device int32 tensor that receives the wait word (a new one when None).
+    Returns (X = B, status). Calls on one device must not overlap: they share the workspace."""
+    if B.dtype != F.dtype or B.dim() != 2 or not B.is_contiguous():
+        raise ValueError("getrs_noalloc: B must be a contiguous (n, k) tensor of the factor's dtype")
+    if status is None:
+        status = torch.zeros(1, dtype=torch.int32, device=B.device)
+    dt = L.C128 if F.dtype == c128 else L.C64
+    L.call('m25x_getrs_noalloc', dt, ptr(F.LU), F.n, F.LU.stride(0), ptr(F.ipiv), ptr(F.perm), ptr(B), B.shape[1],
+           B.shape[1], int(trans), ptr(status), stream())
+    return B, status
+
+
 _FAULTS = {}
 
 
