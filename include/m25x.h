@@ -60,6 +60,35 @@ typedef struct m25x_quad {
     const double *P3, *W3, *P6, *W6;
 } m25x_quad;
 
+/* The layered stack at one angular frequency w (host struct, passed by value to the kernels): N dielectric layers
+ * bottom to top (eps: complex permittivity er (1 - j tand) as (re, im); h: thickness), a PEC ground under layer 1
+ * (bottom_pec = 1) or a half-space of eps_bot, a PEC cover on top of layer N (top_pec = 1) or a half-space of eps_top;
+ * k0 = w / c0, wmu = w mu0, weps = w eps0. Interface k is the top of layer k (0: the bottom). The reference's
+ * LayerStack (src/mom25d/multilayer.py:88-250) at one frequency. */
+#define M25X_MAX_LAYERS 16
+typedef struct m25x_stack {
+    int N, bottom_pec, top_pec, pad;
+    double eps[M25X_MAX_LAYERS][2];
+    double h[M25X_MAX_LAYERS];
+    double eps_bot[2], eps_top[2];
+    double k0, wmu, weps;
+} m25x_stack;
+
+/* Spectral requests of m25x_spectra: M25X_SPEC_NI ints and M25X_SPEC_ND doubles each.
+ *   ints: type, output row, then by type
+ *     M25X_SPEC_TABLE  o, s, nx: G~_A and G~_q between interfaces o and s (two output rows) less nx (0..2) extracted
+ *                      parts, doubles (d, Re cA, Im cA, Re cq, Im cq) per part: c / sqrt(rho^2 + d^2) taken out as
+ *                      2 pi c exp(-kr d) / kr (src/mom25d/multilayer.py:253-318)
+ *     M25X_SPEC_VIA_D  k, lo, hi: the via -> interface kernel of a chain from interface lo to hi seen on interface k
+ *                      (src/mom25d/via.py:196-203)
+ *     M25X_SPEC_VIA_R  m_lo, m_hi, n_lo, n_hi: the via -> via reaction remainder between two chains
+ *                      (src/mom25d/via.py:208-213) */
+#define M25X_SPEC_TABLE 0
+#define M25X_SPEC_VIA_D 1
+#define M25X_SPEC_VIA_R 2
+#define M25X_SPEC_NI 8
+#define M25X_SPEC_ND 10
+
 const char *m25x_last_error(void);
 int m25x_version(void);
 /* Makes `device` current and caches its properties; called once per device before the others. */
@@ -96,6 +125,31 @@ int m25x_near_scatter(void *Z, int zdtype, long ld, const m25x_mesh *g, const in
 int m25x_near_coo(const m25x_mesh *g, const int *P, const int *Q, const double *M, long n, double w, int *ri,
                   int *ci, double *vv, void *stream);
 
+/* ---- matrix-free products (the refinement's FP64 operator without Z) ---------------------------------- */
+
+/* Workspace bytes of m25x_matvec_far for Nt triangles and k columns. */
+size_t m25x_matvec_far_work_bytes(int Nt, int k);
+
+/* Y = Z_far X for the Ne x Ne edge block's centroid-rule far field, without forming Z: the sources aggregated per
+ * triangle, one kernel evaluation per triangle pair for all columns. X, Y: Ne x k complex128 row-major (leading
+ * dimensions ldx, ldy >= k), 1 <= k <= 8. Deterministic (fixed summation orders). Replaces matvec_far and
+ * matvec_far_multi (src/mom25d/core/mom25d_core.cpp:214-314). */
+int m25x_matvec_far(const double *X, int k, long ldx, const m25x_mesh *g, const m25x_tables *tb, double w, double *Y,
+                    long ldy, void *work, size_t work_bytes, void *stream);
+
+/* vals[e] (complex128) = the near-field correction of near entry e, its contributions (as for m25x_near_scatter)
+ * summed in their fixed order: with the entries' rows and columns, the correction as a sparse matrix (the
+ * reference's dZ_exact, src/mom25d/fast.py:571-586). */
+int m25x_near_values(const m25x_mesh *g, const int *P, const int *Q, const double *M, long nentry, const int *estart,
+                     const int *contrib, double w, double *vals, void *stream);
+
+/* Y (+)= A X for a CSR matrix (rowptr: nrow + 1 int32, col int32, vals complex128), X and Y complex128 row-major
+ * with k columns (add = 1 adds to Y, 0 overwrites it). With m25x_near_values' entries: near_matvec
+ * (src/mom25d/core/mom25d_core.cpp:539-556), deterministic (one thread a row and column, the row's entries in
+ * order). */
+int m25x_csr_matvec(const int *rowptr, const int *col, const double *vals, long nrow, const double *X, int k, long ldx,
+                    double *Y, long ldy, int add, void *stream);
+
 /* Near triangle pairs rho^2 + dz^2 < (fac max(size_p, size_q))^2, sorted by (p, q). Two calls: with P == NULL
  * the count is written to *count (synchronous); then the pairs (int32) are written. Replaces near_pairs
  * (src/mom25d/core/mom25d_core.cpp:1216-1263). counts: Nt + 1 int64 of device scratch. */
@@ -118,6 +172,19 @@ int m25x_radial_table_sum(const double *px, const double *py, const double *W, l
  * 1430-1454; src/mom25d/via.py:90-101). */
 int m25x_hankel(const double *rho, int nr, const double *kr, int nk, const double *W, int ns, double *out,
                 void *stream);
+
+/* The spectra of nreq requests at the npts nodes of a Sommerfeld contour (kr, dk: npts complex128 nodes and
+ * weights, device), each times kr dk (the Hankel weights of m25x_hankel): W (nrows x npts complex128, device), a
+ * request's values in its output row(s). req_i / req_d: the requests on the device; req_i_host: the same ints on the
+ * host (checked before the launch). The stack's transmission-line network in the reflection-coefficient form,
+ * overflow-free on the lifted contour. Replaces LayerStack._network / voltage / spectral, TMNetwork and the spectra
+ * of GreenTableML and ViaKernels (src/mom25d/multilayer.py:183-250, 253-308; src/mom25d/via.py:127-213). */
+int m25x_spectra(const m25x_stack *st, double w, const double *kr, const double *dk, int npts, const int *req_i,
+                 const double *req_d, int nreq, const int *req_i_host, double *W, long nrows, void *stream);
+
+/* G~_A and G~_q between interfaces o and s at one complex kr, on the host (synchronous): out = (Re G~_A, Im G~_A,
+ * Re G~_q, Im G~_q). The extraction coefficients' single evaluations (src/mom25d/multilayer.py:279-282, 288-291). */
+int m25x_spectral_point(const m25x_stack *st, double w, double kr_re, double kr_im, int o, int s, double *out);
 
 /* J_n(z), complex z, n = 0, 1, 2, elementwise. Replaces bessel_jn_complex
  * (src/mom25d/core/mom25d_core.cpp:1510-1558). */

@@ -19,6 +19,18 @@ class Tables(C.Structure):
                 ('tab', C.c_void_p), ('kind', C.c_void_p)]
 
 
+MAX_LAYERS = 16
+SPEC_TABLE, SPEC_VIA_D, SPEC_VIA_R = 0, 1, 2
+SPEC_NI, SPEC_ND = 8, 10
+
+
+class Stack(C.Structure):
+    _fields_ = [('N', C.c_int), ('bottom_pec', C.c_int), ('top_pec', C.c_int), ('pad', C.c_int),
+                ('eps', (C.c_double * 2) * MAX_LAYERS), ('h', C.c_double * MAX_LAYERS),
+                ('eps_bot', C.c_double * 2), ('eps_top', C.c_double * 2),
+                ('k0', C.c_double), ('wmu', C.c_double), ('weps', C.c_double)]
+
+
 class Mesh(C.Structure):
     _fields_ = [('Ne', C.c_int), ('Nt', C.c_int), ('ep', C.c_void_p), ('em', C.c_void_p), ('lt', C.c_void_p),
                 ('te', C.c_void_p), ('ap', C.c_void_p), ('am', C.c_void_p), ('vp', C.c_void_p), ('vm', C.c_void_p),
@@ -40,10 +52,16 @@ _SIGS = {
     'm25x_near_scatter': ([P, I, L, P, P, P, P, L, P, P, P, P, D, P], I),
     'm25x_near_coo': ([P, P, P, P, L, D, P, P, P, P], I),
     'm25x_near_pairs': ([P, P, P, I, D, P, C.POINTER(C.c_long), P, P, P], I),
+    'm25x_matvec_far_work_bytes': ([I, I], S),
+    'm25x_matvec_far': ([P, I, L, P, P, D, P, L, P, S, P], I),
+    'm25x_near_values': ([P, P, P, P, L, P, P, D, P, P], I),
+    'm25x_csr_matvec': ([P, P, P, L, P, I, L, P, L, I, P], I),
     'm25x_fan_q': ([P, P, P, I, P, P], I),
     'm25x_radial_table_sum': ([P, P, P, L, L, P, P, L, P, L, D, P, P], I),
     'm25x_hankel': ([P, I, P, I, P, I, P, P], I),
     'm25x_bessel_jn_complex': ([I, P, L, P, P], I),
+    'm25x_spectra': ([P, D, P, P, I, P, P, I, P, P, L, P], I),
+    'm25x_spectral_point': ([P, D, D, D, I, I, P], I),
     'm25x_scale_c64': ([P, L, L, P, P, L, P], I),
     'm25x_diag_scales': ([P, L, L, P, P], I),
     'm25x_getrf_work_bytes': ([I, L], S),

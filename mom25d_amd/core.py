@@ -101,6 +101,33 @@ def near_q(g, tb, quad, P, Q, w):
     return _near(g, tb, quad, P, Q, w)[3].cpu().numpy()
 
 
+def near_matvec(x, g, tb, quad, P, Q, w):
+    """y = dZ x: the near-field correction applied to x (Ne,) without storing it in Z (mom25d_core.cpp:539-556)."""
+    x = np.asarray(x, complex)
+    if x.ndim != 1 or len(x) != len(g['ep']):
+        raise ValueError("near_matvec: x must be (Ne,)")
+    m, plan, M, _ = _near(g, tb, quad, P, Q, w)
+    vals = dv.near_values(m, plan, M, w)
+    return dv.csr_matvec(plan, vals, dv.on_dev(x, dv.c128)).cpu().numpy()
+
+
+def matvec_far(x, g, tb, w):
+    """y = Z_far x for one column, without forming Z (mom25d_core.cpp:214-252)."""
+    x = np.asarray(x, complex)
+    if x.ndim != 1 or len(x) != len(g['ep']):
+        raise ValueError("matvec_far: x must be (Ne,)")
+    return dv.matvec_far(dv.on_dev(x, dv.c128), _mesh(g), _tabs(tb), w).cpu().numpy()
+
+
+def matvec_far_multi(X, g, tb, w):
+    """Y = Z_far X for X (Ne, k), one kernel evaluation per triangle pair for all columns (mom25d_core.cpp:
+    257-314)."""
+    X = np.asarray(X, complex)
+    if X.ndim != 2 or X.shape[0] != len(g['ep']):
+        raise RuntimeError("matvec_far_multi: X must be (Ne, k)")
+    return dv.matvec_far(dv.on_dev(X, dv.c128), _mesh(g), _tabs(tb), w).cpu().numpy()
+
+
 def near_pairs(cent, size, z, fac):
     """(P, Q) int64 near triangle pairs, sorted by (p, q) (mom25d_core.cpp:1216-1263)."""
     P, Q = dv.near_pairs(np.asarray(cent, float), np.asarray(size, float), np.asarray(z, float), float(fac))

@@ -568,6 +568,173 @@ __global__ void k_diag_scales(const double2 *Z, long n, long ld, double *d)
 
 inline unsigned blocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// ---- the far product without Z (matvec_far, matvec_far_multi) ---------------------------------------------
+// Three passes: the sources of each triangle aggregated (per column c: s0 = sum a x_e, s1 = sum a x_e (c_t - v_e)
+// over its edges, a = alpha_e area_t), one kernel evaluation per triangle pair for all k columns (each workgroup
+// a block of observation triangles against one slice of the sources, partial sums kept per slice), and each
+// edge's two halves from the slices summed in slice order. Deterministic: no atomics, fixed orders.
+constexpr int MV_P = 128;      // observation triangles a workgroup, one a thread
+constexpr int MV_Q = 64;       // source triangles staged in LDS a step
+constexpr int MV_KMAX = 8;     // columns a call
+
+// slices of the sources: enough workgroups to fill the chip (about 1024), each slice a multiple of MV_Q
+inline void mv_slices(int Nt, int &nslice, int &qper)
+{
+    const int pblocks = (Nt + MV_P - 1) / MV_P;
+    int want = (1024 + pblocks - 1) / pblocks;
+    want = want < 1 ? 1 : (want > 64 ? 64 : want);
+    qper = (Nt + want - 1) / want;
+    qper = (qper + MV_Q - 1) / MV_Q * MV_Q;
+    if (qper < MV_Q) qper = MV_Q;
+    nslice = (Nt + qper - 1) / qper;
+    if (nslice < 1) nslice = 1;
+}
+
+// S[(t k + c) 3 + {0, 1, 2}] = (s0, s1x, s1y); the triangle's edges in increasing index (the reference's edge loop
+// adds them in that order); act[t]: some edge has t as a half
+__global__ void k_mv_sources(DMesh g, const double2 *X, long ldx, int k, double2 *S, int *act)
+{
+    const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (idx >= (long)g.Nt * k) return;
+    const int t = (int)(idx / k), c = (int)(idx % k);
+    int e3[3], n = 0;
+    for (int i = 0; i < 3; ++i) {
+        const int e = g.te[3 * t + i];
+        if (e >= 0) e3[n++] = e;
+    }
+    for (int i = 1; i < n; ++i)
+        for (int j = i; j > 0 && e3[j - 1] > e3[j]; --j) {
+            const int x = e3[j]; e3[j] = e3[j - 1]; e3[j - 1] = x;
+        }
+    double2 s0 = {0, 0}, sx = {0, 0}, sy = {0, 0};
+    const double cx = g.cent[2 * t], cy = g.cent[2 * t + 1];
+    for (int j = 0; j < n; ++j) {
+        const int e = e3[j];
+        const bool plus = g.ep[e] == t;
+        const double f = (plus ? g.ap[e] : g.am[e]) * g.area[t];
+        const double *v = (plus ? g.vp : g.vm) + 2 * (size_t)e;
+        const double2 xe = X[(size_t)e * ldx + c];
+        const double2 a = make_double2(f * xe.x, f * xe.y);
+        s0 = cadd(s0, a);
+        sx = cadd(sx, cscale(a, cx - v[0]));
+        sy = cadd(sy, cscale(a, cy - v[1]));
+    }
+    S[3 * idx] = s0; S[3 * idx + 1] = sx; S[3 * idx + 2] = sy;
+    if (c == 0) act[t] = n > 0;
+}
+
+// part[((slice Nt + p) k + c) 3 + {0, 1, 2}] = sum over the slice's active q of (G_A s1x, G_A s1y, G_q s0)
+template <int KM>
+__global__ __launch_bounds__(MV_P) void k_mv_pairs(DMesh g, DTab T, const double2 *S, const int *act, int k,
+                                                   int qper, double2 *part)
+{
+    __shared__ double2 sS[MV_Q][3 * KM];
+    __shared__ double2 sc[MV_Q];
+    __shared__ int sl[MV_Q];
+    const int p = blockIdx.x * MV_P + threadIdx.x;
+    const int q0 = blockIdx.y * qper, q1 = min(g.Nt, q0 + qper);
+    const bool live = p < g.Nt;
+    const int lp = live ? g.lt[p] : 0;
+    const double px = live ? g.cent[2 * p] : 0.0, py = live ? g.cent[2 * p + 1] : 0.0;
+    double2 ax[KM], ay[KM], aq[KM];
+#pragma unroll
+    for (int c = 0; c < KM; ++c) ax[c] = ay[c] = aq[c] = make_double2(0, 0);
+    for (int qb = q0; qb < q1; qb += MV_Q) {
+        const int nq = min(MV_Q, q1 - qb);
+        for (int i = threadIdx.x; i < nq * 3 * k; i += MV_P) {
+            const int j = i / (3 * k), r = i % (3 * k);
+            sS[j][r] = S[(size_t)(qb + j) * 3 * k + r];
+        }
+        for (int j = threadIdx.x; j < nq; j += MV_P) {
+            sc[j] = make_double2(g.cent[2 * (qb + j)], g.cent[2 * (qb + j) + 1]);
+            sl[j] = act[qb + j] ? g.lt[qb + j] : -1;
+        }
+        __syncthreads();
+        if (live) {
+            for (int j = 0; j < nq; ++j) {
+                const int lq = sl[j];
+                if (lq < 0) continue;
+                const double dx = px - sc[j].x, dy = py - sc[j].y;
+                double2 ga, gq;
+                tab_kern(T, lp, lq, sqrt(dx * dx + dy * dy), ga, gq);
+#pragma unroll
+                for (int c = 0; c < KM; ++c) {
+                    if (c < k) {
+                        ax[c] = cadd(ax[c], cmul(ga, sS[j][3 * c + 1]));
+                        ay[c] = cadd(ay[c], cmul(ga, sS[j][3 * c + 2]));
+                        aq[c] = cadd(aq[c], cmul(gq, sS[j][3 * c]));
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!live) return;
+    double2 *o = part + ((size_t)blockIdx.y * g.Nt + p) * k * 3;
+#pragma unroll
+    for (int c = 0; c < KM; ++c)
+        if (c < k) { o[3 * c] = ax[c]; o[3 * c + 1] = ay[c]; o[3 * c + 2] = aq[c]; }
+}
+
+// Y[e, c] = sum over the halves (t, alpha, v) of alpha area_t [jw ((c_t - v) . vA_t) + 4 wQ_t / (jw)]
+// (mom25d_core.cpp:300-311), vA and wQ summed over the slices in order
+__global__ void k_mv_out(DMesh g, const double2 *part, int nslice, int k, double w, double2 *Y, long ldy)
+{
+    const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (idx >= (long)g.Ne * k) return;
+    const int e = (int)(idx / k), c = (int)(idx % k);
+    const double iw = 1.0 / w;
+    double2 y = {0, 0};
+    for (int h = 0; h < 2; ++h) {
+        const int t = h ? g.em[e] : g.ep[e];
+        const double f = (h ? g.am[e] : g.ap[e]) * g.area[t];
+        const double *v = (h ? g.vm : g.vp) + 2 * (size_t)e;
+        const double dx = g.cent[2 * t] - v[0], dy = g.cent[2 * t + 1] - v[1];
+        double2 vx = {0, 0}, vy = {0, 0}, wq = {0, 0};
+        for (int sl = 0; sl < nslice; ++sl) {
+            const double2 *o = part + (((size_t)sl * g.Nt + t) * k + c) * 3;
+            vx = cadd(vx, o[0]); vy = cadd(vy, o[1]); wq = cadd(wq, o[2]);
+        }
+        const double2 u = make_double2(dx * vx.x + dy * vy.x, dx * vx.y + dy * vy.y);
+        // jw u + 4 wQ / (jw)
+        const double2 z = make_double2(-w * u.y + 4.0 * wq.y * iw, w * u.x - 4.0 * wq.x * iw);
+        y = cadd(y, cscale(z, f));
+    }
+    Y[(size_t)e * ldy + c] = y;
+}
+
+// ---- the near correction as a sparse operator ------------------------------------------------------------
+// vals[e] = the correction of near entry e (its contributions in their fixed order, as k_near_scatter adds them)
+__global__ __launch_bounds__(256) void k_near_values(DMesh g, const int *P, const int *Q, const double2 *M,
+                                                     long nentry, const int *estart, const int *contrib, double w,
+                                                     double2 *vals)
+{
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= nentry) return;
+    const double iw = 1.0 / w;
+    double2 acc = {0, 0};
+    for (int c = estart[e]; c < estart[e + 1]; ++c) {
+        const int code = contrib[c];
+        const int i = code / 9, s = (code / 3) % 3, u = code % 3;
+        acc = cadd(acc, slot_value(g, P[i], Q[i], s, u, M + 7 * (size_t)i, w, iw));
+    }
+    vals[e] = acc;
+}
+
+// Y[r, c] (+)= sum over the entries of row r of vals[e] X[col[e], c]; one thread a (row, column)
+__global__ void k_csr_matvec(const int *rowptr, const int *col, const double2 *vals, long nrow, const double2 *X,
+                             int k, long ldx, double2 *Y, long ldy, int add)
+{
+    const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (idx >= nrow * k) return;
+    const long r = idx / k;
+    const int c = (int)(idx % k);
+    double2 acc = add ? Y[r * ldy + c] : make_double2(0, 0);
+    double2 sum = {0, 0};
+    for (int e = rowptr[r]; e < rowptr[r + 1]; ++e) sum = cadd(sum, cmul(vals[e], X[(size_t)col[e] * ldx + c]));
+    Y[r * ldy + c] = cadd(acc, sum);
+}
+
 }  // namespace
 
 extern "C" {
@@ -628,6 +795,70 @@ int m25x_near_scatter(void *Z, int zdtype, long ld, const m25x_mesh *g, const in
     else
         return fail(M25X_EINVAL, "near_scatter: zdtype %d", zdtype);
     return check_launch("near_scatter");
+}
+
+size_t m25x_matvec_far_work_bytes(int Nt, int k)
+{
+    if (Nt <= 0 || k <= 0) return 256;
+    int nslice, qper;
+    mv_slices(Nt, nslice, qper);
+    return sizeof(double2) * 3 * (size_t)Nt * k * (1 + (size_t)nslice) + sizeof(int) * (size_t)Nt + 256;
+}
+
+int m25x_matvec_far(const double *X, int k, long ldx, const m25x_mesh *g, const m25x_tables *tb, double w, double *Y,
+                    long ldy, void *work, size_t work_bytes, void *stream)
+{
+    if (!X || !Y || !g || !tb || k < 1 || k > MV_KMAX || ldx < k || ldy < k || !(w > 0))
+        return fail(M25X_EINVAL, "matvec_far: bad arguments (1 <= k <= %d, ldx, ldy >= k, w > 0)", MV_KMAX);
+    if (g->Ne == 0) return M25X_OK;
+    if (!work || work_bytes < m25x_matvec_far_work_bytes(g->Nt, k))
+        return fail(M25X_EINVAL, "matvec_far: workspace of %zu bytes, %zu needed", work_bytes,
+                    m25x_matvec_far_work_bytes(g->Nt, k));
+    hipStream_t s = (hipStream_t)stream;
+    int nslice, qper;
+    mv_slices(g->Nt, nslice, qper);
+    double2 *S = (double2 *)work, *part = S + 3 * (size_t)g->Nt * k;
+    int *act = (int *)(part + 3 * (size_t)g->Nt * k * nslice);
+    DMesh m = dmesh(g);
+    hipLaunchKernelGGL(k_mv_sources, dim3(blocks((long)g->Nt * k, 256)), dim3(256), 0, s, m, (const double2 *)X, ldx,
+                       k, S, act);
+    if (int rc = check_launch("matvec_far")) return rc;
+    const dim3 grid((g->Nt + MV_P - 1) / MV_P, nslice);
+    if (k <= 1)
+        hipLaunchKernelGGL(k_mv_pairs<1>, grid, dim3(MV_P), 0, s, m, dtab(tb), (const double2 *)S, (const int *)act,
+                           k, qper, part);
+    else if (k <= 4)
+        hipLaunchKernelGGL(k_mv_pairs<4>, grid, dim3(MV_P), 0, s, m, dtab(tb), (const double2 *)S, (const int *)act,
+                           k, qper, part);
+    else
+        hipLaunchKernelGGL(k_mv_pairs<MV_KMAX>, grid, dim3(MV_P), 0, s, m, dtab(tb), (const double2 *)S,
+                           (const int *)act, k, qper, part);
+    if (int rc = check_launch("matvec_far")) return rc;
+    hipLaunchKernelGGL(k_mv_out, dim3(blocks((long)g->Ne * k, 256)), dim3(256), 0, s, m, (const double2 *)part,
+                       nslice, k, w, (double2 *)Y, ldy);
+    return check_launch("matvec_far");
+}
+
+int m25x_near_values(const m25x_mesh *g, const int *P, const int *Q, const double *M, long nentry, const int *estart,
+                     const int *contrib, double w, double *vals, void *stream)
+{
+    if (!g || nentry < 0 || (nentry && (!P || !Q || !M || !estart || !contrib || !vals)))
+        return fail(M25X_EINVAL, "near_values: bad arguments");
+    if (nentry == 0) return M25X_OK;
+    hipLaunchKernelGGL(k_near_values, dim3(blocks(nentry, 256)), dim3(256), 0, (hipStream_t)stream, dmesh(g), P, Q,
+                       (const double2 *)M, nentry, estart, contrib, w, (double2 *)vals);
+    return check_launch("near_values");
+}
+
+int m25x_csr_matvec(const int *rowptr, const int *col, const double *vals, long nrow, const double *X, int k, long ldx,
+                    double *Y, long ldy, int add, void *stream)
+{
+    if (!rowptr || nrow < 0 || k < 1 || ldx < k || ldy < k || !X || !Y)
+        return fail(M25X_EINVAL, "csr_matvec: bad arguments");
+    if (nrow == 0) return M25X_OK;
+    hipLaunchKernelGGL(k_csr_matvec, dim3(blocks(nrow * k, 256)), dim3(256), 0, (hipStream_t)stream, rowptr, col,
+                       (const double2 *)vals, nrow, (const double2 *)X, k, ldx, (double2 *)Y, ldy, add);
+    return check_launch("csr_matvec");
 }
 
 int m25x_near_coo(const m25x_mesh *g, const int *P, const int *Q, const double *M, long n, double w, int *ri, int *ci,

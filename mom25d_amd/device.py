@@ -62,6 +62,34 @@ class DevTables:
         # which parts are extracted (the near pairs' kept analytic moments depend on them)
         self.sig = self.kind.cpu().numpy().tobytes() + self.dimg.cpu().numpy().tobytes() + \
             self.d.cpu().numpy().tobytes()
+        self._struct()
+
+    @classmethod
+    def packed(cls, AQ, scal, tab, kind):
+        """From AQ (ntab, nfine, 4) float64 on the device and the per-table scalars on the host (dict of arrays
+        dr, d, dimg, cA, cq, cAi, cqi; tab, kind int32): one upload of the scalars, no read-back."""
+        t = cls.__new__(cls)
+        dv = device()
+        t.AQ = AQ.to(dv, f64).contiguous()
+        t.ntab, t.nfine = int(t.AQ.shape[0]), int(t.AQ.shape[1])
+        n = t.ntab
+        real = np.concatenate([scal['dr'], scal['d'], scal['dimg']]).astype(np.float64)
+        cplx = np.concatenate([scal[k] for k in ('cA', 'cq', 'cAi', 'cqi')]).astype(np.complex128)
+        ints = np.concatenate([np.asarray(tab, np.int32).ravel(), np.asarray(kind, np.int32)])
+        blob = torch.from_numpy(np.concatenate([real, cplx.view(np.float64)])).to(dv)
+        t.dr, t.d, t.dimg = blob[:n], blob[n:2 * n], blob[2 * n:3 * n]
+        cx = blob[3 * n:].view(4, n, 2)
+        t.cA, t.cq, t.cAi, t.cqi = cx[0], cx[1], cx[2], cx[3]
+        iblob = torch.from_numpy(ints).to(dv)
+        t.L = int(np.asarray(tab).shape[0])
+        t.tab, t.kind = iblob[:t.L * t.L].view(t.L, t.L), iblob[t.L * t.L:]
+        t.sig = np.asarray(kind, np.int32).tobytes() + np.asarray(scal['dimg'], np.float64).tobytes() + \
+            np.asarray(scal['d'], np.float64).tobytes()
+        t._host_scal = scal
+        t._struct()
+        return t
+
+    def _struct(self):
         s = L.Tables()
         s.ntab, s.nfine, s.L = self.ntab, self.nfine, self.L
         s.AQ = self.AQ.data_ptr()
@@ -153,6 +181,9 @@ class NearPlan:
             self.contrib = self.erow = self.ecol = torch.zeros(0, dtype=i32, device=dv)
             self.estart = torch.zeros(1, dtype=i32, device=dv)
         self.nentry = int(self.erow.shape[0])
+        # the entries of each row (they are sorted by row, then column): the correction as a CSR matrix
+        self.rowptr = torch.searchsorted(self.erow.long(), torch.arange(Ne + 1, device=dv, dtype=torch.int64),
+                                         right=False).to(i32).contiguous()
         self.stat = torch.zeros((n, 12), dtype=f64, device=dv) if keep_stat else None
         self.stat_ready = False
         self.stat_kind = None
@@ -181,6 +212,51 @@ def near_scatter(Z, mesh, plan, M, w):
     L.call('m25x_near_scatter', ptr(Z), L.C128 if Z.dtype == c128 else L.C64, Z.stride(0), C.byref(mesh.struct),
            ptr(plan.P), ptr(plan.Q), ptr(M), plan.nentry, ptr(plan.erow), ptr(plan.ecol), ptr(plan.estart),
            ptr(plan.contrib), float(w), stream())
+
+
+def near_values(mesh, plan, M, w):
+    """The correction of every near entry (plan.nentry complex128, the rows / columns plan.erow / plan.ecol)."""
+    vals = torch.empty(plan.nentry, dtype=c128, device=device())
+    L.call('m25x_near_values', C.byref(mesh.struct), ptr(plan.P), ptr(plan.Q), ptr(M), plan.nentry, ptr(plan.estart),
+           ptr(plan.contrib), float(w), ptr(vals), stream())
+    return vals
+
+
+def csr_matvec(plan, vals, X, Y=None):
+    """Y (+)= dZ X with dZ the near correction (plan's rows and columns, vals from near_values); X (Ne, k) or
+    (Ne,). Adds into Y when given."""
+    one = X.dim() == 1
+    X = X.reshape(X.shape[0], -1).to(c128).contiguous()
+    n, k = X.shape[0], X.shape[1]
+    if n < len(plan.rowptr) - 1:
+        raise ValueError("csr_matvec: X has fewer rows than the edge block")
+    add = Y is not None
+    out = Y.reshape(Y.shape[0], -1) if add else torch.empty((len(plan.rowptr) - 1, k), dtype=c128, device=X.device)
+    if not out.is_contiguous() or out.dtype != c128:
+        raise ValueError("csr_matvec: Y must be a contiguous complex128 tensor")
+    L.call('m25x_csr_matvec', ptr(plan.rowptr), ptr(plan.ecol), ptr(vals), len(plan.rowptr) - 1, ptr(X), k, k,
+           ptr(out), k, int(add), stream())
+    return out[:, 0] if one and not add else out
+
+
+def matvec_far(X, mesh, tabs, w):
+    """Y = Z_far X (device complex128 (Ne,) or (Ne, k)) without forming Z; columns 8 at a time."""
+    one = X.dim() == 1
+    X = X.reshape(X.shape[0], -1).to(c128)
+    if X.shape[0] != mesh.Ne:
+        raise ValueError("matvec_far: X must have Ne rows")
+    k = X.shape[1]
+    Y = torch.empty((mesh.Ne, k), dtype=c128, device=X.device)
+    for c0 in range(0, k, 8):
+        kc = min(8, k - c0)
+        Xc = X[:, c0:c0 + kc].contiguous()
+        Yc = torch.empty((mesh.Ne, kc), dtype=c128, device=X.device)
+        wb = L.lib().m25x_matvec_far_work_bytes(mesh.Nt, kc)
+        work = torch.empty(wb, dtype=torch.uint8, device=X.device)
+        L.call('m25x_matvec_far', ptr(Xc), kc, kc, C.byref(mesh.struct), C.byref(tabs.struct), float(w), ptr(Yc), kc,
+               ptr(work), wb, stream())
+        Y[:, c0:c0 + kc] = Yc
+    return Y[:, 0] if one else Y
 
 
 def near_coo(mesh, plan, M, w):

@@ -66,19 +66,23 @@ class FastOperator:
         self.sim.green_tables(f)
         self.tabs = self.sim.dev_tables
 
-    def fill(self, f):
-        """Z (device, complex128, N x N) at f, not equilibrated."""
+    def fill(self, f, dtype=dv.c128):
+        """Z (device, N x N, not equilibrated) at f: complex128, or complex64 for the lean path (each entry computed
+        in FP64 and rounded once; the near correction added in FP32). The parts the matrix-free operator needs are
+        kept for this frequency: the near entries' values, the via blocks B and C and the lumped entries."""
         sim, w = self.sim, 2 * np.pi * f
         Ne, Nv = sim.Ne, sim.Nv
         N = Ne + Nv
         self.setup(f)
         dev = dv.device()
-        with sim.log.timer('direct fill', f"N={N}, complex128, GPU"):
-            Z = torch.empty((N, N), dtype=dv.c128, device=dev)
+        with sim.log.timer('direct fill', f"N={N}, {'complex128' if dtype == dv.c128 else 'complex64'}, GPU"):
+            Z = torch.empty((N, N), dtype=dtype, device=dev)
             dv.fill_far(Z, self.mesh, self.tabs, w)
         with sim.log.timer('near-field correction'):
             M, qc = dv.near_moments(self.mesh, self.tabs, self.plan, w)
             dv.near_scatter(Z, self.mesh, self.plan, M, w)
+            self._op = dict(f=f, w=w, vals=dv.near_values(self.mesh, self.plan, M, w) if dtype == dv.c64 else None,
+                            B=None, C=None, lumped=None, M=M)
         if Nv:
             with sim.log.timer('via kernels', f"{Nv} chains"):
                 QF = dv.fan_q(self.mesh, self.tabs, sim.F)
@@ -89,11 +93,37 @@ class FastOperator:
                 Z[:Ne, Ne:] = B
                 Z[Ne:, :Ne] = B.T
                 Z[Ne:, Ne:] = C
+                self._op.update(B=B, C=C)
         lum = sim.lumped_entries(f)
         if lum is not None:
             r, c, v = (torch.from_numpy(x).to(dev) for x in lum)
-            Z.index_put_((r, c), v.to(dv.c128), accumulate=True)
+            v = v.to(dv.c128)
+            Z.index_put_((r, c), v.to(dtype), accumulate=True)
+            self._op['lumped'] = (r, c, v)
         return Z
+
+    def matvec_exact(self, X):
+        """Z X in FP64 without Z (the reference's matvec_exact, fast.py:571-586) at the frequency of the last fill:
+        the far field triangle-aggregated (m25x_matvec_far), the near correction as a sparse matrix, the via rows
+        and columns and the lumped elements. X: device (N,) or (N, k)."""
+        op, Ne = self._op, self.sim.Ne
+        one = X.dim() == 1
+        X = X.reshape(X.shape[0], -1).to(dv.c128)
+        if op.get('vals') is None:
+            op['vals'] = dv.near_values(self.mesh, self.plan, op['M'], op['w'])
+        Y = torch.empty_like(X)
+        Xe = X[:Ne].contiguous()
+        Ye = dv.matvec_far(Xe, self.mesh, self.tabs, op['w'])
+        dv.csr_matvec(self.plan, op['vals'], Xe, Ye)
+        Y[:Ne] = Ye
+        if self.sim.Nv:
+            B, C, Xv = op['B'], op['C'], X[Ne:]
+            Y[:Ne] += B @ Xv
+            Y[Ne:] = B.T @ Xe + C @ Xv
+        if op['lumped'] is not None:
+            r, c, v = op['lumped']
+            Y.index_add_(0, r, v[:, None] * X[c])
+        return Y[:, 0] if one else Y
 
     # ---- solve ---------------------------------------------------------------------------------------------
     def _refine(self, A, B, M, restart, maxiter, stall=None):
@@ -137,6 +167,34 @@ class FastOperator:
         sim = self.sim
         near_failed = precision == 'mixed' and f <= MIXED_RETRY * self._mixed_failed
         t_fresh = time.time()
+        if precision == 'mixed' and self.lean and not near_failed:
+            X = self._solve_lean(f, E, t_fresh)
+        else:
+            X = self._solve_dense(f, E, precision, near_failed, t_fresh)
+        if dv.solve_faults(dv.device()):
+            raise RuntimeError(f"a triangular solve at {f / 1e9:.4g} GHz ran out of a wait bound between its "
+                               "workgroups: its result is not the solution")
+        return X.cpu().numpy()
+
+    @property
+    def lean(self):
+        """The lean mixed path (the reference's streamed solve, fast.py:1120-1224, in this build's form): only the
+        complex64 Z is stored (8 N^2 bytes instead of 24 N^2), and the refinement multiplies by the FP64 Z without
+        forming it (matvec_exact). sim.stream 'on' asks for it."""
+        return getattr(self.sim, 'stream', 'off') == 'on'
+
+    def _solve_lean(self, f, E, t_fresh):
+        Z64 = self.fill(f, dv.c64)
+        dev = Z64.device
+        dsc = 1.0 / torch.sqrt(Z64.diagonal().abs().to(dv.f64) + 1e-300)
+        d32 = dsc.to(torch.float32)
+        Z64.mul_(d32[:, None]).mul_(d32[None, :])               # D Z D in complex64, in place
+        B = dsc[:, None] * torch.from_numpy(np.asarray(E, complex)).to(dev)
+        A = lambda V: dsc[:, None] * self.matvec_exact(dsc[:, None] * V)
+        return self._solve_kept(f, None, dsc, B, t_fresh, fresh='mixed', A=A, Z64=Z64)
+
+    def _solve_dense(self, f, E, precision, near_failed, t_fresh):
+        sim = self.sim
         Z = self.fill(f)
         dev = Z.device
         Ed = torch.from_numpy(np.asarray(E, complex)).to(dev)
@@ -151,10 +209,7 @@ class FastOperator:
                 X = dsc[:, None] * dv.getrs(F, B, trans=1)
         else:
             X = self._solve_kept(f, Z, dsc, B, t_fresh, fresh='double' if near_failed else 'mixed')
-        if dv.solve_faults(dev):
-            raise RuntimeError(f"a triangular solve at {f / 1e9:.4g} GHz ran out of a wait bound between its "
-                               "workgroups: its result is not the solution")
-        return X.cpu().numpy()
+        return X
 
     def _check_info(self, F):
         info = int(F.info.item())
@@ -163,16 +218,17 @@ class FastOperator:
         if info:
             self.sim.log.note(f"WARNING the LU met an exactly zero pivot at {info - 1}: the matrix is singular")
 
-    def _solve_kept(self, f, Z, dsc, B, t_fresh, fresh='mixed'):
+    def _solve_kept(self, f, Z, dsc, B, t_fresh, fresh='mixed', A=None, Z64=None):
         """Mixed precision with factors kept on the GPU (fast.py:996-1084): a kept factor of a nearby frequency
         first; else a fresh complex64 one (fresh='mixed') or, within MIXED_RETRY of a frequency where complex64 did not
         converge, a fresh complex128 one (fresh='double'). A complex128 factor is kept like a complex64 one, so the
         points after a failed one are preconditioned by it instead of each taking its own complex128 LU."""
         sim = self.sim
-        n = Z.shape[0]
-        Z64 = torch.empty((n, n), dtype=dv.c64, device=Z.device) if fresh == 'mixed' else None
-        dv.scale_c64(Z, dsc, Z64)                               # Z <- D Z D (kept, FP64), and its complex64 copy
-        A = lambda V: dv.zgemm_nk(Z, V)
+        n = len(dsc)
+        if A is None:                                           # the dense path: Z (FP64) kept for the products
+            Z64 = torch.empty((n, n), dtype=dv.c64, device=Z.device) if fresh == 'mixed' else None
+            dv.scale_c64(Z, dsc, Z64)                           # Z <- D Z D (kept, FP64), and its complex64 copy
+            A = lambda V: dv.zgemm_nk(Z, V)
         k = self._reusable(f)
         if k is not None:
             f0, F0, d0, tf0 = k
@@ -209,6 +265,9 @@ class FastOperator:
         sim.log.note(f"WARNING mixed precision did not converge at {f / 1e9:.4g} GHz (residual {rel:.1e}) - solving "
                      f"in double precision up to {MIXED_RETRY * f / 1e9:.4g} GHz")
         self._mixed_failed = max(self._mixed_failed, f)
+        if Z is None:                                           # the lean path: the FP64 Z filled for this point
+            Z = self.fill(f)
+            dv.scale_c64(Z, dsc)
         return self._fresh_double(f, Z, dsc, B, t_fresh)
 
     def _fresh_double(self, f, Z, dsc, B, t_fresh):

@@ -1,71 +1,28 @@
 """Via (vertical wire) kernels on the device (src/mom25d/via.py:127-235).
 
 A via's current is constant along each chain (the layers between two interfaces where it touches metal); its end
-charges are spread over the fan triangles around the via vertex and enter through G_q like the strips' charges.
-Per frequency: the TM network voltages for every source interface (TMNetwork), the smooth via -> interface kernels
-D(rho) tabulated per (interface, span) through one batched Hankel transform, the via-via remainder R (directly at
-the pair distances, or tabulated above 64 pairs) and the homogeneous K0 part.
+charges spread over the fan triangles around the via vertex and enter through G_q like the strips' charges. Per
+frequency, one m25x_spectra launch gives every spectrum the vias need on the frequency's contour:
+  D(k, span)   the smooth via -> interface kernel of a chain span seen on interface k, tabulated on the uniform
+               grid (one Hankel transform and one spline product for all of them)
+  R(sm, sn)    the via -> via reaction remainder between two spans, transformed at the pairs' distances (or
+               tabulated, for many pairs)
+and homog() the analytic K0 part of the reaction, on the host.
 """
 import numpy as np
 import torch
 from scipy.special import kv
 
 from .base import C0, MU0
+from . import _lib as L
 from . import device as dv
-from .multilayer import device_contour, hankel, resample, spline_operator
-
-
-class TMNetwork:
-    """V[s][i](kr): TM voltage at interface i for a unit shunt current at interface s (via.py:127-166)."""
-
-    def __init__(self, stack, kr, w):
-        self.st, self.kr, self.w, N = stack, kr, w, stack.N
-        Y0, tn, sc, Yd, Yu = stack._network(kr, w, 'e')
-        self.k2 = [None] + [(w / C0) ** 2 * stack.eps(j) for j in range(1, N + 1)]
-        self.kz2 = [None] + [k2 - kr * kr for k2 in self.k2[1:]]
-        self.V = {}
-        for s in stack.metal_interfaces:
-            v = [None] * (N + 1)
-            v[s] = 1.0 / (Yu[s] + Yd[s])
-            for k in range(s + 1, N + 1):
-                v[k] = torch.zeros_like(kr) if stack.is_pec(k) else v[k - 1] * sc[k] / (1 + 1j * (Yu[k] / Y0[k]) * tn[k])
-            for k in range(s, 0, -1):
-                v[k - 1] = torch.zeros_like(kr) if stack.is_pec(k - 1) else \
-                    v[k] * sc[k] / (1 + 1j * (Yd[k - 1] / Y0[k]) * tn[k])
-            self.V[s] = v
-
-    def is_pec(self, k):
-        return self.st.is_pec(k)
-
-    def dV(self, i, s):
-        return 0.0 if self.is_pec(s) else self.V[s][i] - self.V[s][i - 1]
-
-    def S(self, k, layers):
-        return sum((self.dV(j, k) / self.kz2[j] for j in layers), 0.0)
-
-    def P(self, li, lj, with_homog=True):
-        out = 0.0
-        kr2 = self.kr * self.kr
-        for i in li:
-            for j in lj:
-                out = out + kr2 / (self.kz2[i] * self.kz2[j]) * (self.dV(i, j) - self.dV(i, j - 1))
-                if with_homog and i == j:
-                    out = out + 1j * self.w * MU0 * self.st.layers[j - 1].h / (kr2 - self.k2[j])
-        return out
-
-
-def chain_ends(k_bot, k_top):
-    return [(k_bot, +1.0), (k_top, -1.0)]
-
-
-def _layers(span):
-    return range(span[0] + 1, span[1] + 1)
+from .multilayer import NFINE, Requests, device_contour, hankel, quad_grid, resample, spline_operator
 
 
 class DevTable1D:
     """A radial table on the uniform grid of nfine points (values on the device), linear lookup (via.py:111-123)."""
 
-    def __init__(self, v, rho_max, nfine=8192):
+    def __init__(self, v, rho_max, nfine=NFINE):
         self.v, self.rho_max, self.nfine = v, rho_max, nfine
         self._dr = np.linspace(0, rho_max, nfine)[1]
 
@@ -77,64 +34,61 @@ class DevTable1D:
         return v[i] * (1 - t) + v[i + 1] * t
 
 
+def span_layers(span):
+    """The dielectric layers a chain from interface span[0] up to span[1] runs through."""
+    return range(span[0] + 1, span[1] + 1)
+
+
 class ViaKernels:
     """Per-frequency via kernels for chain spans [(k_bot, k_top), ...] and the metal interfaces (via.py:179-235)."""
 
     def __init__(self, f, stack, spans, interfaces, rho_max, nrho=200):
-        w = 2 * np.pi * f
-        kr, dk, _ = device_contour(f, stack, rho_max)            # the contour of the frequency's tables
-        stack = stack.at(f)
-        net = TMNetwork(stack, kr, w)
-        self.f, self.st, self.w, self.kr, self.dk, self.net = f, stack, w, kr, dk, net
-        self._gq = {}
+        self.f, self.w = f, 2 * np.pi * f
+        self.st = stack.at(f)
+        self.contour = device_contour(f, stack, rho_max)
         self.rho_max, self.nrho, self.D = rho_max, nrho, {}
-        self.rho = dv.on_dev(rho_max * np.linspace(0, 1, nrho) ** 2, dv.f64)
-        keys, specs = [], []
-        for span in sorted(set(spans)):
+        self.spans = sorted(set(spans))
+        req, d_rows, self._r_rows = Requests(), [], {}
+        for span in self.spans:
             for k in interfaces:
-                spec = net.S(k, _layers(span))
-                for ke, div in chain_ends(*span):
-                    spec = spec - div * self.gq(k, ke) / (1j * w)
-                keys.append((k, span))
-                specs.append(spec * torch.ones_like(kr))
-        if keys:
-            vals = hankel(torch.stack(specs), kr, dk, self.rho)              # one transform for every table
-            fine = resample(spline_operator(rho_max, nrho, 8192), vals.T)     # (8192, ntables)
-            for j, key in enumerate(keys):
+                d_rows.append(((k, span), req.add(L.SPEC_VIA_D, (k, span[0], span[1]))))
+        for sm in self.spans:
+            for sn in self.spans:
+                self._r_rows[(sm, sn)] = req.add(L.SPEC_VIA_R, (sm[0], sm[1], sn[0], sn[1]))
+        self._W = req.run(self.st, self.w, self.contour)
+        if d_rows:
+            rows = torch.tensor([r for _, r in d_rows], device=self._W.device)
+            grid = dv.on_dev(quad_grid(rho_max, nrho), dv.f64)
+            fine = resample(spline_operator(rho_max, nrho, NFINE), hankel(self._W[rows], self.contour, grid).T)
+            for j, (key, _) in enumerate(d_rows):
                 self.D[key] = DevTable1D(fine[:, j].contiguous(), rho_max)
 
-    def gq(self, a, b):
-        """G~_q between interfaces a and b (0 at a PEC one), cached per pair."""
-        if self.net.is_pec(a) or self.net.is_pec(b):
-            return 0.0
-        key = (min(a, b), max(a, b))
-        if key not in self._gq:
-            self._gq[key] = self.st.spectral(self.kr, self.w, a, b)[1]
-        return self._gq[key]
-
-    def _R_spec(self, sm, sn):
-        spec = self.net.P(_layers(sm), _layers(sn), with_homog=False)
-        for ka, da in chain_ends(*sm):
-            for kb, db in chain_ends(*sn):
-                spec = spec - da * db * self.gq(ka, kb) / (1j * self.w)
-        return spec * torch.ones_like(self.kr)
+    def _r_row(self, sm, sn):
+        row = self._r_rows.get((tuple(sm), tuple(sn)))
+        if row is None:
+            raise KeyError(f"no via reaction between spans {sm} and {sn} at this frequency")
+        return self._W[row:row + 1]
 
     def R(self, sm, sn, rho):
         """The via-via reaction remainder at distances rho (host array in, host array out)."""
         rho = np.atleast_1d(np.asarray(rho, float))
-        return hankel(self._R_spec(sm, sn)[None], self.kr, self.dk, dv.on_dev(rho, dv.f64))[0].cpu().numpy()
+        return hankel(self._r_row(sm, sn), self.contour, dv.on_dev(rho, dv.f64))[0].cpu().numpy()
 
     def R_table(self, sm, sn):
-        vals = hankel(self._R_spec(sm, sn)[None], self.kr, self.dk, self.rho)      # (1, nrho)
-        return DevTable1D(resample(spline_operator(self.rho_max, self.nrho, 8192), vals.T)[:, 0].contiguous(),
+        """R tabulated on the uniform grid (one transform on the quadratic grid): for many pairs of one span pair."""
+        grid = dv.on_dev(quad_grid(self.rho_max, self.nrho), dv.f64)
+        vals = hankel(self._r_row(sm, sn), self.contour, grid)
+        return DevTable1D(resample(spline_operator(self.rho_max, self.nrho, NFINE), vals.T)[:, 0].contiguous(),
                           self.rho_max)
 
     def homog(self, sm, sn, rho):
-        """The analytic K0 part of the reaction between spans (host; a handful of values per board)."""
-        rho, out = np.atleast_1d(rho), np.zeros(np.size(rho), complex)
-        for j in set(_layers(sm)) & set(_layers(sn)):
-            kj = np.sqrt(self.net.k2[j] + 0j)
-            if kj.imag > 0:
-                kj = -kj
+        """The analytic part of the reaction of two spans through their shared layers j: a line current in the
+        homogeneous medium of layer j, (j w mu0 h_j / 2 pi) K0(j k_j rho), k_j with Im k_j <= 0 (host)."""
+        rho = np.atleast_1d(np.asarray(rho, float))
+        out = np.zeros(rho.shape, complex)
+        k0sq = (self.w / C0) ** 2
+        for j in sorted(set(span_layers(sm)) & set(span_layers(sn))):
+            kj = np.sqrt(k0sq * self.st.eps(j) + 0j)
+            kj = -kj if kj.imag > 0 else kj
             out += 1j * self.w * MU0 * self.st.layers[j - 1].h / (2 * np.pi) * kv(0, 1j * kj * rho)
         return out

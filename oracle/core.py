@@ -47,7 +47,7 @@ class _Mesh(C.Structure):
 class _NearOut(C.Structure):
     _fields_ = [('mode', C.c_int), ('Z', C.c_void_p), ('ld', C.c_long), ('single', C.c_int),
                 ('ri', C.c_void_p), ('ci', C.c_void_p), ('vv', C.c_void_p), ('qcorr', C.c_void_p),
-                ('stat', C.c_void_p), ('stat_ready', C.c_int)]
+                ('stat', C.c_void_p), ('stat_ready', C.c_int), ('x', C.c_void_p), ('y', C.c_void_p)]
 
 
 def _c(a, dt):
@@ -151,6 +151,10 @@ def _near(mode, g, tb, q, P, Q, w, Z=None):
         if not (Z.flags.c_contiguous and Z.dtype in (np.complex64, np.complex128)):
             raise ValueError("near_into_dense: Z must be a C-ordered complex64 or complex128 array")
         o.Z, o.ld, o.single = _p(Z), Z.shape[1], int(Z.dtype == np.complex64)
+    elif mode == 1:
+        x = _c(Z, complex)
+        out['x'], out['y'] = x, np.zeros(len(g['ep']), complex)
+        o.x, o.y = _p(x), _p(out['y'])
     elif mode == 2:
         out['ri'], out['ci'] = np.zeros(9 * n, np.int32), np.zeros(9 * n, np.int32)
         out['vv'] = np.zeros(9 * n, complex)
@@ -170,6 +174,29 @@ def _near(mode, g, tb, q, P, Q, w, Z=None):
 def near_into_dense(Z, g, tb, q, P, Q, w):
     """Z += the near-field correction (atomic adds); returns qcorr (mom25d_core.cpp:520-537)."""
     return _near(0, g, tb, q, P, Q, w, Z)[0]
+
+
+def near_matvec(x, g, tb, q, P, Q, w):
+    """y = dZ x, the near-field correction applied to x (Ne,) without storing it (atomic adds; mom25d_core.cpp:
+    539-556)."""
+    return _near(1, g, tb, q, P, Q, w, x)[1]['y']
+
+
+def matvec_far_multi(X, g, tb, w):
+    """Y = Z_far X for X (Ne, k), triangle-aggregated, without Z (mom25d_core.cpp:257-314)."""
+    h = _Hold()
+    X = _c(X, complex)
+    if X.ndim != 2 or X.shape[0] != len(g['ep']):
+        raise RuntimeError("matvec_far_multi: X must be (Ne, k)")
+    Y = np.zeros(X.shape, complex)
+    m, t = _mesh(g, h), _tables(tb, h)
+    _check(lib().orc_matvec_far_multi(_p(X), C.c_int(X.shape[1]), C.byref(m), C.byref(t), C.c_double(w), _p(Y)))
+    return Y
+
+
+def matvec_far(x, g, tb, w):
+    """y = Z_far x for one column (mom25d_core.cpp:214-252)."""
+    return matvec_far_multi(np.asarray(x, complex).reshape(-1, 1), g, tb, w)[:, 0]
 
 
 def near_coo(g, tb, q, P, Q, w):

@@ -6,11 +6,13 @@
  * Every function follows a reference function (paths relative to the reference root):
  *   orc_fill_far          src/mom25d/core/mom25d_core.cpp:132-173   (fill_far_t: centroid rule, symmetric)
  *   orc_far_block         src/mom25d/core/mom25d_core.cpp:176-211   (far_block)
+ *   orc_matvec_far_multi  src/mom25d/core/mom25d_core.cpp:214-314   (matvec_far, matvec_far_multi: Z_far X
+ *                                                                     triangle-aggregated, without Z)
  *   tab_rest / tab_kern   src/mom25d/core/mom25d_core.cpp:64-90     (Tables::rest, Tables::kern)
  *   orc_offplane_one      src/mom25d/core/mom25d_core.cpp:331-365   (offplane_one; sing.py:19-73)
  *   rule_over_p           src/mom25d/core/mom25d_core.cpp:321-324   (sing.py:76-85, RULE_TIE)
  *   orc_offplane_pairs    src/mom25d/core/mom25d_core.cpp:367-387
- *   orc_near              src/mom25d/core/mom25d_core.cpp:417-575   (near_core, modes dense / coo / q)
+ *   orc_near              src/mom25d/core/mom25d_core.cpp:417-594   (near_core, modes dense / matvec / coo / q)
  *   orc_near_pairs        src/mom25d/core/mom25d_core.cpp:1216-1263 (grid search, sorted by (p, q))
  *   orc_radial_table_sum  src/mom25d/core/mom25d_core.cpp:1269-1298
  *   orc_hankel_real       src/mom25d/core/mom25d_core.cpp:1430-1454 (glibc j0)
@@ -135,6 +137,68 @@ int orc_fill_far(void *Zv, int single, long ld, const orc_mesh *g, const orc_tab
     return 0;
 }
 
+/* ---- Y = Z_far X without Z (mom25d_core.cpp:214-314): the sources aggregated per triangle (s0 = sum a A x,
+ * s1 = sum a A x (c - v), edges in order), one kernel evaluation per triangle pair for all k columns (X, Y: Ne x k,
+ * row-major), then each edge's two halves. k = 1 is matvec_far (which skips a q whose three sums are zero, where
+ * matvec_far_multi skips a q no edge touches: the same sums). ---- */
+int orc_matvec_far_multi(const cd *X, int K, const orc_mesh *g, const orc_tables *tb, double w, cd *Y)
+{
+    const int Ne = g->Ne, Nt = g->Nt;
+    const cd jw = I * w, inv_jw = 1.0 / (I * w);
+    cd *s0 = calloc((size_t)Nt * K, sizeof(cd)), *s1x = calloc((size_t)Nt * K, sizeof(cd));
+    cd *s1y = calloc((size_t)Nt * K, sizeof(cd));
+    cd *vAx = malloc(sizeof(cd) * (size_t)Nt * K), *vAy = malloc(sizeof(cd) * (size_t)Nt * K);
+    cd *wQ = malloc(sizeof(cd) * (size_t)Nt * K);
+    char *act = calloc((size_t)Nt, 1);
+    if (!s0 || !s1x || !s1y || !vAx || !vAy || !wQ || !act) {
+        free(s0); free(s1x); free(s1y); free(vAx); free(vAy); free(wQ); free(act);
+        return fail("matvec_far: out of memory");
+    }
+    for (int e = 0; e < Ne; ++e) {
+        const int p = g->ep[e], q = g->em[e];
+        for (int c = 0; c < K; ++c) {
+            const cd xe = X[(size_t)e * K + c];
+            const cd a = g->ap[e] * g->area[p] * xe, b = g->am[e] * g->area[q] * xe;
+            const size_t ip = (size_t)p * K + c, iq = (size_t)q * K + c;
+            s0[ip] += a; s1x[ip] += a * (g->cent[2 * p] - g->vp[2 * e]); s1y[ip] += a * (g->cent[2 * p + 1] - g->vp[2 * e + 1]);
+            s0[iq] += b; s1x[iq] += b * (g->cent[2 * q] - g->vm[2 * e]); s1y[iq] += b * (g->cent[2 * q + 1] - g->vm[2 * e + 1]);
+        }
+        act[p] = act[q] = 1;
+    }
+#pragma omp parallel
+    {
+        cd *ax = malloc(sizeof(cd) * K), *ay = malloc(sizeof(cd) * K), *aq = malloc(sizeof(cd) * K);
+#pragma omp for schedule(dynamic, 16)
+        for (int p = 0; p < Nt; ++p) {
+            for (int c = 0; c < K; ++c) ax[c] = ay[c] = aq[c] = 0;
+            for (int q = 0; q < Nt; ++q) {
+                if (!act[q]) continue;
+                cd ga, gq;
+                tab_kern(tb, g->lt[p], g->lt[q], cdist(g->cent, p, q), &ga, &gq);
+                const cd *S0 = s0 + (size_t)q * K, *SX = s1x + (size_t)q * K, *SY = s1y + (size_t)q * K;
+                for (int c = 0; c < K; ++c) { ax[c] += ga * SX[c]; ay[c] += ga * SY[c]; aq[c] += gq * S0[c]; }
+            }
+            for (int c = 0; c < K; ++c) {
+                vAx[(size_t)p * K + c] = ax[c]; vAy[(size_t)p * K + c] = ay[c]; wQ[(size_t)p * K + c] = aq[c];
+            }
+        }
+        free(ax); free(ay); free(aq);
+    }
+    for (int e = 0; e < Ne; ++e) {
+        const int p = g->ep[e], q = g->em[e];
+        const double dpx = g->cent[2 * p] - g->vp[2 * e], dpy = g->cent[2 * p + 1] - g->vp[2 * e + 1];
+        const double dqx = g->cent[2 * q] - g->vm[2 * e], dqy = g->cent[2 * q + 1] - g->vm[2 * e + 1];
+        for (int c = 0; c < K; ++c) {
+            const size_t ip = (size_t)p * K + c, iq = (size_t)q * K + c;
+            const cd yp = g->ap[e] * g->area[p] * (jw * (dpx * vAx[ip] + dpy * vAy[ip]) + 4.0 * wQ[ip] * inv_jw);
+            const cd ym = g->am[e] * g->area[q] * (jw * (dqx * vAx[iq] + dqy * vAy[iq]) + 4.0 * wQ[iq] * inv_jw);
+            Y[(size_t)e * K + c] = yp + ym;
+        }
+    }
+    free(s0); free(s1x); free(s1y); free(vAx); free(vAy); free(wQ); free(act);
+    return 0;
+}
+
 /* ---- far field for arbitrary row / column edge lists (mom25d_core.cpp:176-211) ---- */
 int orc_far_block(const int *R, int nr, const int *C, int nc, const orc_mesh *g, const orc_tables *tb, double w,
                   cd *o)
@@ -242,6 +306,7 @@ typedef struct {
     void *Z; long ld; int single;
     int *ri, *ci; cd *vv; cd *qcorr;
     double *stat; int stat_ready;
+    const cd *x; cd *y;
 } orc_nearout;
 
 int orc_near(const orc_mesh *g, const orc_tables *tb, const double *nodes, const int *tris, const double *P3,
@@ -332,6 +397,13 @@ int orc_near(const orc_mesh *g, const orc_tables *tb, const double *nodes, const
 #pragma omp atomic
                         z[1] += cimag(val);
                     }
+                } else if (o->mode == 1) {      /* y[em] += val x[en] (mom25d_core.cpp:539-556) */
+                    cd t = val * o->x[en];
+                    double *yy = (double *)(o->y + em_);
+#pragma omp atomic
+                    yy[0] += creal(t);
+#pragma omp atomic
+                    yy[1] += cimag(t);
                 } else if (o->mode == 2) {
                     o->ri[slot] = em_; o->ci[slot] = en; o->vv[slot] = val;
                 }

@@ -1,7 +1,14 @@
 """The device Green's tables and via kernels against the reference's own values (tests/golden/tables_c1_c2.npz,
 written by tests/golden/make_tables.py from the reference's multilayer.py / via.py with its compiled core): the
 microstrip stack (C1) and the seeded 4-layer stack (C2) at 1, 3.7 and 8 GHz, every interface pair: A_rest, Q_rest
-and D to 1e-10 of each table's largest value, R + homog to 1e-10 of each value, the extraction scalars to 1e-12."""
+and D to 1e-10 of each table's largest value, R + homog to 1e-10 of each value, the extraction scalars to 1e-12.
+
+One exception, the split of a same-interface table with its nearest image extracted: the image's coefficients
+(cAi, cqi) are the difference of the static spectra at k1 = 18 / dimg and their direct part, which is e^18 times
+larger, so any evaluation of the network rounds them to ~1e-8 relative (the reference's own values carry that
+error, and this build's network, in reflection-coefficient form, rounds differently). What enters Z is the table
+plus the extracted parts: for such a table the total kernel, rest + c_img / sqrt(rho^2 + dimg^2), is compared to 1e-10
+of its largest value, and the image coefficients to 1e-6."""
 import numpy as np
 import pytest
 
@@ -24,11 +31,17 @@ def test_device_tables_match_reference(gpu, name):
             tabs = green_tables(f, st, ifs, rho_max, dmax)
             for (a, b), t in tabs.items():
                 key = f'{name}/f{fi}/t{a}{b}'
-                assert rel(t.A_rest.cpu().numpy()[idx], G[key + '/A']) < 1e-10, key
-                assert rel(t.Q_rest.cpu().numpy()[idx], G[key + '/Q']) < 1e-10, key
                 sc = G[key + '/scal']
+                A, Q = t.A_rest.cpu().numpy()[idx], t.Q_rest.cpu().numpy()[idx]
+                gA, gQ = G[key + '/A'], G[key + '/Q']
                 mine = np.array([t.d, t.dimg, t.cA, t.cq, t.cAi, t.cqi], complex)
-                assert np.all(np.abs(mine - sc[:6]) <= 1e-12 * np.maximum(np.abs(sc[:6]), 1e-300)), key
+                tol = np.array([1e-12] * 4 + [1e-6 if t.dimg else 1e-12] * 2)
+                assert np.all(np.abs(mine - sc[:6]) <= tol * np.maximum(np.abs(sc[:6]), 1e-300)), key
+                if t.dimg:
+                    img = 1.0 / np.hypot(t.rho[idx], t.dimg)
+                    A, Q, gA, gQ = A + t.cAi * img, Q + t.cqi * img, gA + sc[4] * img, gQ + sc[5] * img
+                assert rel(A, gA) < 1e-10, key
+                assert rel(Q, gQ) < 1e-10, key
             vk = ViaKernels(f, st, spans, ifs, rho_max)
             for (k, span), tab in vk.D.items():
                 key = f'{name}/f{fi}/D{k}_{span[0]}{span[1]}'

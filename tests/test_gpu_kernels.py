@@ -99,6 +99,31 @@ def test_near_field_matches_oracle(gpu, case):
     assert np.abs(core.near_q(g, tb, q, P32, Q32, w) - co).max() <= 1e-12 * np.abs(co).max()
 
 
+def test_matrix_free_products_match_oracle(gpu, case):
+    """matvec_far / matvec_far_multi / near_matvec through the drop-in module against the oracle's (which equal the
+    reference's compiled kernels bit for bit, test_oracle_pins): relative 1e-12 of each column's largest value
+    (the triangle-pair sums run in slices on the GPU, in one sequence on the CPU), for k = 1, 3 and 11 columns
+    (two calls of at most 8); and the far product equal to the dense far fill times X."""
+    from mom25d_amd import core
+    g, q, tb, w, st = case['g'], case['q'], case['tb'], case['w'], case['stack']
+    Ne = len(g['ep'])
+    rng = np.random.default_rng(11)
+    for k in (1, 3, 11):
+        X = rng.standard_normal((Ne, k)) + 1j * rng.standard_normal((Ne, k))
+        Yg, Yo = core.matvec_far_multi(X, g, tb, w), oc.matvec_far_multi(X, g, tb, w)
+        assert np.abs(Yg - Yo).max() <= 1e-12 * np.abs(Yo).max(), k
+    x = X[:, 4]
+    yg, yo = core.matvec_far(x, g, tb, w), oc.matvec_far(x, g, tb, w)
+    assert np.abs(yg - yo).max() <= 1e-12 * np.abs(yo).max()
+    Z = np.zeros((Ne, Ne), complex)
+    core.fill_far(Z, g, tb, w)
+    assert np.abs(yg - Z @ x).max() <= 1e-12 * np.abs(Z @ x).max()
+    P, Q = oc.near_pairs(case['rwg'].cent, case['size'], st.z[case['tl']], 5.0)
+    P32, Q32 = P.astype(np.int32), Q.astype(np.int32)
+    yn, yno = core.near_matvec(x, g, tb, q, P32, Q32, w), oc.near_matvec(x, g, tb, q, P32, Q32, w)
+    assert np.abs(yn - yno).max() <= 1e-12 * np.abs(yno).max()
+
+
 def test_near_pairs_match_oracle(gpu, case):
     from mom25d_amd import core
     st = case['stack']
