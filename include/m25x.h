@@ -249,6 +249,64 @@ int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, con
 int m25x_getrs_noalloc(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B,
                        long ldb, int nrhs, int trans, int *status, void *stream);
 
+/* ---- batches: many systems in one launch per step ------------------------------------------------------
+ * No counterpart in the reference, whose solver takes one frequency at a time (src/mom25d/run.py:55-64 calls
+ * Simulation.sparams once a point; src/mom25d/fast.py:704-850 factors one Z). These run the same kernels as
+ * m25x_getrf / m25x_getrs / m25x_zgemm_nk over a list of independent systems of any sizes, each launch covering
+ * every system (a grid dimension over the items), so that small systems fill the GPU together and a sweep step is
+ * a fixed sequence of launches (capturable in one HIP graph). Each item's results are those of the one-system call
+ * on the same inputs. items_host: the item table on the host (sizes for the grids, checked); items_dev: the same
+ * table in device memory. */
+
+/* One LU of the batch: as m25x_getrf's arguments (tournament pivoting), work: m25x_getrf_work_bytes(dtype, n) bytes
+ * of device memory of its own (work_bytes: its size). */
+typedef struct m25x_lu_item {
+    void *A;
+    long n, lda;
+    int *ipiv, *perm, *info;
+    void *work;
+    size_t work_bytes;
+} m25x_lu_item;
+
+/* The tournament-pivoting LUs of nitems matrices of the given dtype in one sequence of launches (each panel step
+ * one launch per kernel for all items). Items sorted by n, largest first, make the later steps' grids smaller
+ * (any order is correct). */
+int m25x_getrf_batched(int dtype, int nitems, const m25x_lu_item *items_host, const m25x_lu_item *items_dev,
+                       void *stream);
+
+/* One solve of the batch: as m25x_getrs's arguments (perm required: m25x_getrf's), nrhs <= 4 (complex128) or 8
+ * (complex64), work: m25x_getrs_work_bytes(dtype, n, nrhs) bytes of its own. */
+typedef struct m25x_rs_item {
+    const void *LU;
+    long n, lda;
+    const int *perm;
+    void *B;
+    long ldb;
+    int nrhs, pad;
+    void *work;
+    size_t work_bytes;
+} m25x_rs_item;
+
+/* The solves of nitems factors (trans as m25x_getrs), each triangle one launch for all items: the chains of the
+ * items' diagonal blocks side by side in one grid. status (device int): zeroed, then nonzero if a wait between
+ * workgroups ran out in any item. */
+int m25x_getrs_batched(int dtype, int nitems, const m25x_rs_item *items_host, const m25x_rs_item *items_dev, int trans,
+                       int *status, void *stream);
+
+/* One product of the batch: Y = Z X, Z n x n complex128 row-major (leading dimension ld), X and Y n x k (k <= 8)
+ * with leading dimensions ldx, ldy. */
+typedef struct m25x_mv_item {
+    const void *Z;
+    long n, ld;
+    const void *X;
+    void *Y;
+    long ldx, ldy;
+    int k, pad;
+} m25x_mv_item;
+
+/* The products of nitems items in one launch. */
+int m25x_zgemm_nk_batched(int nitems, const m25x_mv_item *items_host, const m25x_mv_item *items_dev, void *stream);
+
 /* ---- launch timing (benchmarks) ------------------------------------------------------------------------------
  * No counterpart in the reference (its fill and LU are CPU code, LAPACK or a vendor library). m25x_timing(mask)
  * opens a record (clearing the last) in which every launch of a kernel whose bit (1 << M25X_K_*) is in mask is

@@ -31,6 +31,22 @@ class Stack(C.Structure):
                 ('k0', C.c_double), ('wmu', C.c_double), ('weps', C.c_double)]
 
 
+class LuItem(C.Structure):
+    _fields_ = [('A', C.c_void_p), ('n', C.c_long), ('lda', C.c_long), ('ipiv', C.c_void_p), ('perm', C.c_void_p),
+                ('info', C.c_void_p), ('work', C.c_void_p), ('work_bytes', C.c_size_t)]
+
+
+class RsItem(C.Structure):
+    _fields_ = [('LU', C.c_void_p), ('n', C.c_long), ('lda', C.c_long), ('perm', C.c_void_p), ('B', C.c_void_p),
+                ('ldb', C.c_long), ('nrhs', C.c_int), ('pad', C.c_int), ('work', C.c_void_p),
+                ('work_bytes', C.c_size_t)]
+
+
+class MvItem(C.Structure):
+    _fields_ = [('Z', C.c_void_p), ('n', C.c_long), ('ld', C.c_long), ('X', C.c_void_p), ('Y', C.c_void_p),
+                ('ldx', C.c_long), ('ldy', C.c_long), ('k', C.c_int), ('pad', C.c_int)]
+
+
 class Mesh(C.Structure):
     _fields_ = [('Ne', C.c_int), ('Nt', C.c_int), ('ep', C.c_void_p), ('em', C.c_void_p), ('lt', C.c_void_p),
                 ('te', C.c_void_p), ('ap', C.c_void_p), ('am', C.c_void_p), ('vp', C.c_void_p), ('vm', C.c_void_p),
@@ -70,6 +86,9 @@ _SIGS = {
     'm25x_getrs_work_bytes': ([I, L, L], S),
     'm25x_getrs': ([I, P, L, L, P, P, P, L, I, I, P, S, P], I),
     'm25x_getrs_noalloc': ([I, P, L, L, P, P, P, L, I, I, P, P], I),
+    'm25x_getrf_batched': ([I, I, P, P, P], I),
+    'm25x_getrs_batched': ([I, I, P, P, I, P, P], I),
+    'm25x_zgemm_nk_batched': ([I, P, P, P], I),
     'm25x_timing': ([C.c_uint], I),
     'm25x_timing_records': ([P, P, P, P, P, L], L),
     'm25x_kernel_name': ([I], C.c_char_p),

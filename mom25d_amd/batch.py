@@ -1,0 +1,136 @@
+"""Many frequency points - of one board or of many boards of uneven size - solved as one batch on the GPU.
+
+No counterpart in the reference, which solves one point at a time (src/mom25d/run.py:55-64 calls
+Simulation.sparams per frequency; src/mom25d/fast.py:704-850 fills and factors one Z). A batch:
+
+  fill      every point's Z (FP64) on the device, equilibrated (D Z D), and its complex64 copy
+  factor    all complex64 copies in one sequence of launches (m25x_getrf_batched: each panel step one launch per
+            kernel for all points), optionally captured once in a HIP graph and replayed for the next batch of the
+            same sizes
+  refine    one GMRES over every point's ports side by side (columns of one (n_max, k_total) block, zero-padded):
+            the operator is block-diagonal - the FP64 products of all points in one launch (m25x_zgemm_nk_batched),
+            the preconditioner all points' triangular solves in one launch per triangle (m25x_getrs_batched) - and
+            a point whose refinement stalls stops alone
+  S         per point; a point whose complex64 factor does not converge is solved again by its own board's
+            single-point path (complex128 LU)
+
+The same constants as the single-point mixed path (restart 30, stall (4, 0.1), converged at 100 x 1e-6), so each
+point's answer is within the mixed path's bound of its double-precision one.
+"""
+import time
+
+import numpy as np
+import torch
+
+from . import device as dv
+from .fast import FRESH_STALL, MIXED_MAX_ITER, POLISH
+from .linalg import gmres_batched
+
+
+class _Cols:
+    """Which columns of the stacked right-hand sides belong to which point, and the per-point column blocks of the
+    active subset that GMRES passes to the operators."""
+
+    def __init__(self, ks):
+        self.first = np.concatenate([[0], np.cumsum(ks)[:-1]]).astype(int)
+        self.k = np.asarray(ks, int)
+        self.owner = np.repeat(np.arange(len(ks)), ks)
+        self.active = [(int(c), int(k)) for c, k in zip(self.first, self.k)]
+
+    def track(self, act):
+        """Active global columns (ascending) -> each point's (first position, count) among them."""
+        own = self.owner[act]
+        out = []
+        for i in range(len(self.k)):
+            pos = np.flatnonzero(own == i)
+            out.append((int(pos[0]), len(pos)) if len(pos) else (0, 0))
+        self.active = out
+
+
+class SweepBatch:
+    """Points [(sim, f), ...] solved together in mixed precision. sim: a Simulation (its mesh, near plan and tables
+    stay resident on the device); one board may appear at many frequencies and many boards at once."""
+
+    def __init__(self, points, graph=False):
+        self.points = list(points)
+        self.graph = graph
+        self._graph = None
+        self.stats = {}
+
+    def run(self):
+        """S of every point (a list of nports x nports arrays, in the order given)."""
+        t0 = time.perf_counter()
+        pts = self.points
+        Zs, Z64s, ds, Bs, ks = [], [], [], [], []
+        for sim, f in pts:
+            Z = sim.fast.fill(f)
+            d = dv.diag_scales(Z)
+            Z64 = torch.empty(Z.shape, dtype=dv.c64, device=Z.device)
+            dv.scale_c64(Z, d, Z64)                          # Z <- D Z D (FP64, kept), Z64 its complex64 copy
+            E = torch.from_numpy(sim.excitation()).to(Z.device)
+            Zs.append(Z); Z64s.append(Z64); ds.append(d); Bs.append(d[:, None] * E); ks.append(E.shape[1])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        if self.graph:
+            g = torch.cuda.CUDAGraph()
+            F = dv.getrf_batched(Z64s, graph=g)
+            g.replay()
+            self._graph = (g, F)
+        else:
+            F = dv.getrf_batched(Z64s)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        Y, ok = self._refine(Zs, F.factors, Bs, ks)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        out, fallback = [], 0
+        for i, (sim, f) in enumerate(pts):
+            if ok[i]:
+                c0, k = int(self.cols.first[i]), ks[i]
+                I = (ds[i][:, None] * Y[:Zs[i].shape[0], c0:c0 + k]).cpu().numpy()
+            else:
+                fallback += 1
+                I = sim.fast.solve_direct(f, sim.excitation(), precision='double')
+            out.append(sim.sparams_from(f, I))
+        if dv.solve_faults():
+            raise RuntimeError("a batched triangular solve ran out of a wait bound: its results are not solutions")
+        t4 = time.perf_counter()
+        self.stats = dict(points=len(pts), fill_s=t1 - t0, factor_s=t2 - t1, refine_s=t3 - t2, s_s=t4 - t3,
+                          total_s=t4 - t0, fallback=fallback, iterations=self.iters)
+        return out
+
+    def _refine(self, Zs, factors, Bs, ks):
+        """GMRES over all points' columns side by side: (Y (n_max, k_total), converged per point)."""
+        nmax = max(Z.shape[0] for Z in Zs)
+        self.cols = cols = _Cols(ks)
+        B = torch.zeros((nmax, int(cols.k.sum())), dtype=dv.c128, device=Zs[0].device)
+        for i, b in enumerate(Bs):
+            B[:b.shape[0], cols.first[i]:cols.first[i] + ks[i]] = b
+        solve = dv.BatchSolve(factors, cols.active)
+        A = lambda V: dv.zgemm_nk_batched(Zs, V, cols.active)
+        M = lambda R: solve(R, trans=1, subset=cols.active)
+        tol = 1e-6
+        Y, it = gmres_batched(A, B, M=M, rtol=tol, restart=MIXED_MAX_ITER, maxiter=1, stall=FRESH_STALL,
+                              track=cols.track, stall_each=True)
+        cols.track(np.arange(B.shape[1]))
+        R = B - A(Y)
+        rel = (torch.linalg.vector_norm(R, dim=0) /
+               torch.clamp(torch.linalg.vector_norm(B, dim=0), min=1e-300)).cpu().numpy()
+        # answers accepted within 100 x tol, polished by one more cycle when still above POLISH x tol (the single
+        # path's rule), the others left to the single-point fallback
+        okc = np.isfinite(rel) & (rel <= 100 * tol)
+        more = np.flatnonzero(okc & (rel > POLISH * tol))
+        if len(more):
+            sub = torch.from_numpy(more).to(B.device)
+            pc = _Cols(np.bincount(cols.owner[more], minlength=len(ks)))
+            solve2 = dv.BatchSolve(factors, pc.active)
+            A2 = lambda V: dv.zgemm_nk_batched(Zs, V, pc.active)
+            M2 = lambda R: solve2(R, trans=1, subset=pc.active)
+            Y2, it2 = gmres_batched(A2, B[:, sub], M=M2, x0=Y[:, sub], rtol=tol, restart=MIXED_MAX_ITER, maxiter=1,
+                                    track=pc.track, stall_each=True)
+            Y[:, sub] = Y2
+            for c, n in zip(more, it2):
+                it[c] += n
+        self.iters = it
+        ok = [bool(okc[cols.first[i]:cols.first[i] + ks[i]].all()) for i in range(len(ks))]
+        return Y, ok

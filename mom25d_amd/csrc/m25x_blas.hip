@@ -1,5 +1,6 @@
 // m25x_blas.hip - the FP64 kernels of the GMRES refinement: Y = Z X with Z (n x n complex128, row-major) read
 // once from HBM per product and X (n x k, k <= 8) staged through LDS in chunks shared by 16 rows; one build per k.
+#include <algorithm>
 #include "m25x_common.h"
 
 using namespace m25x;
@@ -16,14 +17,16 @@ template <int K> constexpr int rows_per_wave() { return K <= 5 ? 4 : 2; }
 // Each wave takes RPW rows; per chunk of CH columns a lane loads its UJ columns of all RPW rows at once (RPW x UJ
 // independent 16 B loads in flight), reads each column's K values of X once from LDS (laid out [K][CH], so a
 // wave's reads are consecutive) and applies them to the RPW rows.
+// X, Y: leading dimensions ldx, ldy; kk <= K columns in use (the others read as zero and are not written)
 template <int K>
-__global__ __launch_bounds__(256) void k_zgemm_nk(const double2 *__restrict__ Z, long n, long ld,
-                                                  const double2 *__restrict__ X, double2 *__restrict__ Y)
+__device__ __forceinline__ void zgemm_body(const double2 *__restrict__ Z, long n, long ld,
+                                           const double2 *__restrict__ X, long ldx, double2 *__restrict__ Y, long ldy,
+                                           int kk, long bx)
 {
     constexpr int RPW = rows_per_wave<K>();
     __shared__ double2 xs[K][CH];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const long i0 = (long)blockIdx.x * (4 * RPW) + wv * RPW;
+    const long i0 = bx * (4 * RPW) + wv * RPW;
     double2 acc[RPW][K];
 #pragma unroll
     for (int r = 0; r < RPW; ++r)
@@ -33,7 +36,7 @@ __global__ __launch_bounds__(256) void k_zgemm_nk(const double2 *__restrict__ Z,
         const long jn = n - j0 < CH ? n - j0 : CH;
         for (int idx = tid; idx < CH * K; idx += 256) {
             const int t = idx / K, c = idx % K;
-            xs[c][t] = t < jn ? X[(j0 + t) * K + c] : make_double2(0, 0);
+            xs[c][t] = (t < jn && c < kk) ? X[(j0 + t) * ldx + c] : make_double2(0, 0);
         }
         double2 z[RPW][UJ];
 #pragma unroll
@@ -65,8 +68,25 @@ __global__ __launch_bounds__(256) void k_zgemm_nk(const double2 *__restrict__ Z,
         for (int c = 0; c < K; ++c) {
             double x = acc[r][c].x, y = acc[r][c].y;
             for (int o = 32; o > 0; o >>= 1) { x += __shfl_xor(x, o); y += __shfl_xor(y, o); }
-            if (lane == 0 && i0 + r < n) Y[(i0 + r) * K + c] = make_double2(x, y);
+            if (lane == 0 && i0 + r < n && c < kk) Y[(i0 + r) * ldy + c] = make_double2(x, y);
         }
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_zgemm_nk(const double2 *__restrict__ Z, long n, long ld,
+                                                  const double2 *__restrict__ X, double2 *__restrict__ Y)
+{
+    zgemm_body<K>(Z, n, ld, X, K, Y, K, K, blockIdx.x);
+}
+
+// one item a grid row (blockIdx.y); row blocks past the item's n leave at once
+template <int K>
+__global__ __launch_bounds__(256) void k_zgemm_nk_b(const m25x_mv_item *__restrict__ items)
+{
+    const m25x_mv_item it = items[blockIdx.y];
+    if ((long)blockIdx.x * (4 * rows_per_wave<K>()) >= it.n) return;
+    zgemm_body<K>((const double2 *)it.Z, it.n, it.ld, (const double2 *)it.X, it.ldx, (double2 *)it.Y, it.ldy, it.k,
+                  blockIdx.x);
 }
 
 }  // namespace
@@ -90,6 +110,38 @@ extern "C" int m25x_zgemm_nk(const double *Z, long n, long ld, const double *X, 
 #undef M25X_NK
     }
     return check_launch("zgemm_nk");
+}
+
+extern "C" int m25x_zgemm_nk_batched(int nitems, const m25x_mv_item *h, const m25x_mv_item *d, void *stream)
+{
+    if (nitems < 0 || (nitems && (!h || !d))) return fail(M25X_EINVAL, "zgemm_nk_batched: bad arguments");
+    long nmax = 0;
+    int kmax = 1;
+    double bytes = 0, flops = 0;
+    for (int i = 0; i < nitems; ++i) {
+        const m25x_mv_item &it = h[i];
+        if (!it.Z || !it.X || !it.Y || it.n < 0 || it.ld < it.n || it.k < 1 || it.k > RK || it.ldx < it.k ||
+            it.ldy < it.k)
+            return fail(M25X_EINVAL, "zgemm_nk_batched: item %d: bad arguments (1 <= k <= %d, ld >= n, ldx, ldy >= k)",
+                        i, RK);
+        nmax = std::max(nmax, it.n);
+        kmax = std::max(kmax, it.k);
+        bytes += 16.0 * it.n * (double)it.n + 32.0 * it.n * it.k;
+        flops += 8.0 * it.n * (double)it.n * it.k;
+    }
+    if (nitems == 0 || nmax == 0) return M25X_OK;
+    hipStream_t s = (hipStream_t)stream;
+    TimedLaunch tl(s, M25X_K_ZGEMM_NK, M25X_C128, bytes, flops);
+    switch (kmax) {
+#define M25X_NKB(K)                                                                                           \
+    case K:                                                                                                   \
+        hipLaunchKernelGGL(k_zgemm_nk_b<K>, dim3((unsigned)((nmax + 4 * rows_per_wave<K>() - 1) / (4 * rows_per_wave<K>())), \
+                                                 (unsigned)nitems), dim3(256), 0, s, d);                      \
+        break;
+        M25X_NKB(1) M25X_NKB(2) M25X_NKB(3) M25X_NKB(4) M25X_NKB(5) M25X_NKB(6) M25X_NKB(7) M25X_NKB(8)
+#undef M25X_NKB
+    }
+    return check_launch("zgemm_nk_batched");
 }
 
 // ---- classical Gram-Schmidt, twice, for the batched GMRES ------------------------------------------------------

@@ -239,12 +239,12 @@ __global__ void k_laswp(T *A, long lda, long n, long j0, int jb, const int *ipiv
 // step's reads are consecutive), else read from memory at each step.
 constexpr int TC = 16;
 template <typename T, int JB>
-__global__ __launch_bounds__(256) void k_trsm(T *A, long lda, long n, long j0, int jb, long c0)
+__device__ __forceinline__ void trsm_body(T *A, long lda, long n, long j0, int jb, long c0, long bx)
 {
     constexpr bool STAGE = JB <= TW;
     __shared__ T X[JB][TC + 1];
     __shared__ T Lt[STAGE ? TW : 1][TW];
-    const long cb = c0 + (long)blockIdx.x * TC;
+    const long cb = c0 + bx * TC;
     const int nc = (int)std::min((long)TC, n - cb);
     const int tid = threadIdx.x;
     for (int idx = tid; idx < jb * TC; idx += 256) {
@@ -275,6 +275,12 @@ __global__ __launch_bounds__(256) void k_trsm(T *A, long lda, long n, long j0, i
     }
 }
 
+template <typename T, int JB>
+__global__ __launch_bounds__(256) void k_trsm(T *A, long lda, long n, long j0, int jb, long c0)
+{
+    trsm_body<T, JB>(A, lda, n, j0, jb, c0, blockIdx.x);
+}
+
 // ---- C -= A B (SUB) or C = A B on the matrix cores -------------------------------------------------------
 // C (M x N), A (M x K), B (K x N), column-major, complex. C = A B may overwrite A or B in place when a workgroup
 // tile's output depends on nothing another workgroup writes (M <= 64 with C = B, or N <= 64 with C = A): every
@@ -284,15 +290,15 @@ __global__ __launch_bounds__(256) void k_trsm(T *A, long lda, long n, long j0, i
 constexpr int GM = 64, GN = 64, GK = 8;
 
 template <typename T, bool SUB, typename AC = T>
-__global__ __launch_bounds__(256) void k_gemm(T *C, long ldc, const T *A, long lda, const T *B, long ldb, long M,
-                                              long N, long K)
+__device__ __forceinline__ void gemm_body(T *C, long ldc, const T *A, long lda, const T *B, long ldb, long M, long N,
+                                          long K, long bx, long by)
 {
     typedef typename CX<AC>::R R;           // the matrix cores' type: products and sums in it, one rounding to T
     typedef typename CX<T>::R TR;
     typedef typename CX<AC>::acc4 acc4;
     __shared__ T As[GK][GM];         // As[p][i] = A[i][p]
     __shared__ T Bs[GK][GN + 1];     // Bs[p][j] = B[p][j]
-    const long m0 = (long)blockIdx.x * GM, n0 = (long)blockIdx.y * GN;
+    const long m0 = bx * GM, n0 = by * GN;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;     // wave tile origin in the workgroup tile
     const int li = lane & 15, kk = lane >> 4, part = kk & 1, ph = kk >> 1;
@@ -361,6 +367,13 @@ __global__ __launch_bounds__(256) void k_gemm(T *C, long ldc, const T *A, long l
             }
 }
 
+template <typename T, bool SUB, typename AC = T>
+__global__ __launch_bounds__(256) void k_gemm(T *C, long ldc, const T *A, long lda, const T *B, long ldb, long M,
+                                              long N, long K)
+{
+    gemm_body<T, SUB, AC>(C, ldc, A, lda, B, ldb, M, N, K, blockIdx.x, blockIdx.y);
+}
+
 // ---- the interchanges as a permutation: perm = the swaps of ipiv applied in order to 0..n-1 ------------------
 __global__ void k_perm(const int *ipiv, long n, int *perm)
 {
@@ -400,7 +413,7 @@ constexpr int MAXRHS = 8;
 // upper part of top) in LDS; the 64 columns in two halves of 32 so a row's half stays in registers for either
 // element type.
 template <typename T>
-__global__ __launch_bounds__(256) void k_l21(T *A, long lda, long r0, long m, long j0, int jb, const T *top)
+__device__ __forceinline__ void l21_body(T *A, long lda, long r0, long m, long j0, int jb, const T *top, long bx)
 {
     __shared__ T Us[TW][TW + 1];
     __shared__ T rd[TW];
@@ -412,7 +425,7 @@ __global__ __launch_bounds__(256) void k_l21(T *A, long lda, long r0, long m, lo
     __syncthreads();
     if (tid < TW) rd[tid] = tid < jb ? trecip(Us[tid][tid]) : CX<T>::make(1, 0);
     __syncthreads();
-    const long r = r0 + (long)blockIdx.x * 256 + tid;
+    const long r = r0 + bx * 256 + tid;
     if (r >= r0 + m) return;
     T *row = A + j0 * lda + r;                  // element (r, j0 + c) at row[c * lda]
     auto half = [&](auto hc) __attribute__((always_inline)) {
@@ -456,24 +469,30 @@ __global__ __launch_bounds__(256) void k_l21(T *A, long lda, long r0, long m, lo
     Unroll<0, 2>::run(half);
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_l21(T *A, long lda, long r0, long m, long j0, int jb, const T *top)
+{
+    l21_body<T>(A, lda, r0, m, j0, jb, top, blockIdx.x);
+}
+
 // The panel's row moves on columns [c0, c0 + nc): a column per 128 threads, every source read before any write;
 // the panel's own columns [j0, j0 + jb) then take the root's L11 \ U11 in rows [j0, j0 + jb). With perm, one more
 // workgroup moves the entries of the row permutation (position -> original row) the same way.
 template <typename T>
-__global__ __launch_bounds__(256) void k_apply_moves(T *A, long lda, long c0, long nc, const int *pairs, long j0,
-                                                     int jb, const T *top, int *perm)
+__device__ __forceinline__ void apply_moves_body(T *A, long lda, long c0, long nc, const int *pairs, long j0, int jb,
+                                                 const T *top, int *perm, long bx, long perm_block)
 {
     const int t = threadIdx.x & 127, h = threadIdx.x >> 7;
     const int np = pairs[0];
     const int dst = t < np ? pairs[1 + 2 * t] : 0, src = t < np ? pairs[2 + 2 * t] : 0;
-    if (perm && blockIdx.x == gridDim.x - 1) {      // the extra workgroup: the same moves on the row permutation
+    if (perm && bx == perm_block) {                 // the extra workgroup: the same moves on the row permutation
         int v = 0;
         if (h == 0 && t < np) v = perm[src];
         __syncthreads();
         if (h == 0 && t < np) perm[dst] = v;
         return;
     }
-    for (long cb = (long)blockIdx.x * 8; cb < std::min(nc, (long)blockIdx.x * 8 + 8); cb += 2) {
+    for (long cb = bx * 8; cb < std::min(nc, bx * 8 + 8); cb += 2) {
         const long c = c0 + cb + h;
         const bool on = cb + h < nc;
         T *col = A + c * lda;
@@ -485,6 +504,14 @@ __global__ __launch_bounds__(256) void k_apply_moves(T *A, long lda, long c0, lo
         if (on && c >= j0 && c < j0 + jb)
             for (int i = t; i < jb; i += 128) col[j0 + i] = top[(c - j0) * TW + i];
     }
+}
+
+// grid: blocks(nc, 8) workgroups of columns, and with perm one more (the last) for the permutation
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_moves(T *A, long lda, long c0, long nc, const int *pairs, long j0,
+                                                     int jb, const T *top, int *perm)
+{
+    apply_moves_body<T>(A, lda, c0, nc, pairs, j0, jb, top, perm, blockIdx.x, (long)gridDim.x - 1);
 }
 
 // kind 0: U^T y = b (forward); 1: L^T z = y (backward); 2: L y = b (forward); 3: U x = y (backward).
@@ -605,8 +632,8 @@ __device__ __attribute__((always_inline)) void subst(T (&t)[NW], const T (*Ms)[S
 // (leading dimension nrhs) filled with UNSET words beforehand. abort: raised if a wait exceeds its bound (the result
 // is then garbage, not a hang). NR: 4 (nrhs <= 4) or 8.
 template <typename T, int NR>
-__global__ __launch_bounds__(TB) void k_trsv(const T *LU, long lda, long n, const T *rdg, const T *src, long lds,
-                                             T *dst, int nrhs, int kind, int *abort)
+__device__ __forceinline__ void trsv_body(const T *LU, long lda, long n, const T *rdg, const T *src, long lds, T *dst,
+                                          int nrhs, int kind, int *abort, int q)
 {
     constexpr int NW = NR / 4;               // right-hand sides a solver wave takes
     constexpr int D = TrsvDepth<T>::D;
@@ -614,7 +641,6 @@ __global__ __launch_bounds__(TB) void k_trsv(const T *LU, long lda, long n, cons
     __shared__ T red[NWV][NR][SB];           // the waves' partial sums
     __shared__ T yb[NWV][64];                // each wave's y values of the block it meets
     const int nb = (int)((n + SB - 1) / SB);
-    const int q = blockIdx.x;
     if (q >= nb) return;
     const bool fwd = kind_fwd(kind), rowmajor = kind <= 1, unit = kind == 1 || kind == 2;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -755,6 +781,13 @@ __global__ __launch_bounds__(TB) void k_trsv(const T *LU, long lda, long n, cons
 #pragma unroll
     for (int w = 0; w < NW; ++w)
         if (rr[w] < nrhs && inrow) put(dst + (r0 + lane) * nrhs + rr[w], t[w]);
+}
+
+template <typename T, int NR>
+__global__ __launch_bounds__(TB) void k_trsv(const T *LU, long lda, long n, const T *rdg, const T *src, long lds,
+                                             T *dst, int nrhs, int kind, int *abort)
+{
+    trsv_body<T, NR>(LU, lda, n, rdg, src, lds, dst, nrhs, kind, abort, blockIdx.x);
 }
 
 // 1 / U_ii for the substitutions with U and U^T
@@ -1016,6 +1049,252 @@ int getrs_t(const T *LU, long n, long lda, const int *ipiv, const int *perm_in, 
     return getrs_noreset<T>(LU, n, lda, ipiv, perm_in, B, ldb, nrhs, trans, (char *)work, abort, s);
 }
 
+// ---- batches ------------------------------------------------------------------------------------------------
+// Every kernel of a panel step (or of a solve) once for all items of a batch: grid rows (or the last grid
+// dimension) over the items, each workgroup reading its item from the item table; workgroups past an item's own
+// extent, and items already done, leave at once. The items' results are those of the one-matrix calls.
+
+// the tournament counters zeroed, info cleared, perm = 0 .. n - 1
+template <typename T>
+__global__ void k_lu_init_b(const m25x_lu_item *items)
+{
+    const m25x_lu_item it = items[blockIdx.y];
+    const TourWork<T> w = tour_carve<T>(it.work, it.n);
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < w.nn) w.arrive[i] = 0;
+    if (it.perm && i < it.n) it.perm[i] = (int)i;
+    if (i == 0 && it.info) *it.info = 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_moves_b(const m25x_lu_item *items, long j0)
+{
+    const m25x_lu_item it = items[blockIdx.y];
+    const long n = it.n;
+    if (j0 >= n) return;
+    const long nb = (n + 7) / 8;
+    if ((long)blockIdx.x > nb || ((long)blockIdx.x == nb && !it.perm)) return;
+    const TourWork<T> w = tour_carve<T>(it.work, n);
+    apply_moves_body<T>((T *)it.A, it.lda, 0, n, w.pairs, j0, (int)std::min((long)TW, n - j0), w.top, it.perm,
+                        blockIdx.x, nb);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_l21_b(const m25x_lu_item *items, long j0)
+{
+    const m25x_lu_item it = items[blockIdx.y];
+    const long n = it.n;
+    if (j0 >= n) return;
+    const int jb = (int)std::min((long)TW, n - j0);
+    const long rows = n - j0 - jb;
+    if (rows <= 0 || (long)blockIdx.x * 256 >= rows) return;
+    const TourWork<T> w = tour_carve<T>(it.work, n);
+    l21_body<T>((T *)it.A, it.lda, j0 + jb, rows, j0, jb, (const T *)w.top, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm_b(const m25x_lu_item *items, long j0)
+{
+    const m25x_lu_item it = items[blockIdx.y];
+    const long n = it.n;
+    if (j0 >= n) return;
+    const int jb = (int)std::min((long)TW, n - j0);
+    const long c0 = j0 + jb;
+    if (c0 >= n || (long)blockIdx.x * TC >= n - c0) return;
+    trsm_body<T, TW>((T *)it.A, it.lda, n, j0, jb, c0, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_b(const m25x_lu_item *items, long j0)
+{
+    const m25x_lu_item it = items[blockIdx.z];
+    const long n = it.n;
+    if (j0 >= n) return;
+    const int jb = (int)std::min((long)TW, n - j0);
+    const long c0 = j0 + jb, M = n - c0;
+    if (M <= 0 || (long)blockIdx.x * GM >= M || (long)blockIdx.y * GN >= M) return;
+    T *A = (T *)it.A;
+    const long lda = it.lda;
+    gemm_body<T, true, double2>(A + c0 * lda + c0, lda, A + j0 * lda + c0, lda, A + c0 * lda + j0, lda, M, M, jb,
+                                blockIdx.x, blockIdx.y);
+}
+
+template <typename T>
+int getrf_batched_t(int nitems, const m25x_lu_item *h, const m25x_lu_item *d, hipStream_t s)
+{
+    long nmax = 0, work_max = 1;
+    bool sorted = true;
+    for (int i = 0; i < nitems; ++i) {
+        const m25x_lu_item &it = h[i];
+        if (!it.A || !it.ipiv || !it.work || it.n < 0 || it.lda < it.n || it.n > (1L << 30))
+            return fail(M25X_EINVAL, "getrf_batched: item %d: bad arguments", i);
+        if (it.work_bytes < tour_work_bytes<T>(it.n))
+            return fail(M25X_EINVAL, "getrf_batched: item %d: workspace of %zu bytes, %zu needed", i, it.work_bytes,
+                        tour_work_bytes<T>(it.n));
+        nmax = std::max(nmax, it.n);
+        work_max = std::max(work_max, std::max(it.n, tour_nodes(it.n, Tour<T>::RL, Tour<T>::AR)));
+        if (i && it.n > h[i - 1].n) sorted = false;
+    }
+    if (nitems == 0 || nmax == 0) return M25X_OK;
+    const int dt = dtype_of<T>();
+    const double es = sizeof(T);
+    hipLaunchKernelGGL(k_lu_init_b<T>, dim3(blocks(work_max, 256), nitems), dim3(256), 0, s, d);
+    if (int rc = check_launch("getrf_batched")) return rc;
+    int na = nitems;
+    for (long j0 = 0; j0 < nmax; j0 += TW) {
+        // the items still in the factorisation: a prefix when sorted by size (the grids then shrink with them)
+        while (sorted && na > 0 && h[na - 1].n <= j0) --na;
+        const long nbig = sorted ? h[0].n : nmax, mbig = nbig - j0;
+        double flops = 0, pbytes = 0;
+        for (int i = 0; i < na; ++i) {
+            const long n = h[i].n;
+            if (j0 >= n) continue;
+            const long jb = std::min((long)TW, n - j0), M = n - j0 - jb;
+            flops += 8.0 * M * M * jb;
+            pbytes += es * (n - j0) * jb;
+        }
+        {
+            TimedLaunch tl(s, M25X_K_TSLU, dt, pbytes, 0.0);
+            if (int rc = launch_tslu_b((const T *)nullptr, d, na, j0, (int)tour_leaves(mbig, Tour<T>::RL), s))
+                return rc;
+        }
+        hipLaunchKernelGGL(k_apply_moves_b<T>, dim3(blocks(nbig, 8) + 1, na), dim3(256), 0, s, d, j0);
+        if (mbig > TW) hipLaunchKernelGGL(k_l21_b<T>, dim3(blocks(mbig - TW, 256), na), dim3(256), 0, s, d, j0);
+        if (int rc = check_launch("getrf_batched/panel")) return rc;
+        if (mbig > TW) {
+            hipLaunchKernelGGL((k_trsm_b<T>), dim3(blocks(mbig - TW, TC), na), dim3(256), 0, s, d, j0);
+            TimedLaunch tl(s, M25X_K_GEMM, dt, 0.0, flops);
+            hipLaunchKernelGGL(k_gemm_b<T>, dim3(blocks(mbig - TW, GM), blocks(mbig - TW, GN), na), dim3(256), 0, s, d,
+                               j0);
+        }
+        if (int rc = check_launch("getrf_batched/update")) return rc;
+    }
+    return M25X_OK;
+}
+
+// the solve's buffers in an item's workspace (as getrs_noreset carves them)
+template <typename T> struct RsWork {
+    T *W1, *W2, *W0, *rd;
+};
+template <typename T> __device__ inline RsWork<T> rs_carve(void *work, long n)
+{
+    RsWork<T> w;
+    const size_t wn = (size_t)n * MAXRHS;
+    w.W1 = (T *)work;
+    w.W2 = w.W1 + wn;
+    w.W0 = w.W2 + wn;
+    w.rd = w.W0 + wn;
+    return w;
+}
+
+// W1, W2 filled with UNSET words; with rdiag, 1 / U_ii too
+template <typename T>
+__global__ void k_rs_prep_b(const m25x_rs_item *items, int c0, int rdiag)
+{
+    const m25x_rs_item it = items[blockIdx.y];
+    if (c0 >= it.nrhs) return;
+    const RsWork<T> w = rs_carve<T>(it.work, it.n);
+    const long words = (long)(2 * (size_t)it.n * MAXRHS * sizeof(T) / 8);
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < words) ((unsigned long long *)w.W1)[i] = ~0ull;
+    if (rdiag && i < it.n) {
+        const T *LU = (const T *)it.LU;
+        w.rd[i] = trecip(LU[i * it.lda + i]);
+    }
+}
+
+// the right-hand sides of part c0 in or out through the permutation: mode 0 B -> W0 gathered by perm (trans 0),
+// mode 1 W2 -> B scattered by perm (trans 1), mode 2 W2 -> B as is (trans 0)
+template <typename T>
+__global__ void k_rs_move_b(const m25x_rs_item *items, int c0, int NR, int mode)
+{
+    const m25x_rs_item it = items[blockIdx.y];
+    if (c0 >= it.nrhs) return;
+    const int kc = std::min(NR, it.nrhs - c0);
+    const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (idx >= it.n * kc) return;
+    const long i = idx / kc;
+    const int r = (int)(idx % kc);
+    const RsWork<T> w = rs_carve<T>(it.work, it.n);
+    T *B = (T *)it.B + c0;
+    if (mode == 0) w.W0[i * kc + r] = B[(long)it.perm[i] * it.ldb + r];
+    else if (mode == 1) B[(long)it.perm[i] * it.ldb + r] = w.W2[i * kc + r];
+    else B[i * it.ldb + r] = w.W2[i * kc + r];
+}
+
+// one triangle of every item's part c0: workgroup g is block g / nitems of item g % nitems (the chains side by
+// side; a workgroup waits only on blocks of its own item with smaller indices, which have smaller g)
+template <typename T, int NR>
+__global__ __launch_bounds__(TB) void k_trsv_b(const m25x_rs_item *items, int nitems, int c0, int kind, int *abort)
+{
+    const int item = blockIdx.x % nitems, q = blockIdx.x / nitems;
+    const m25x_rs_item it = items[item];
+    if (c0 >= it.nrhs || q >= (int)((it.n + SB - 1) / SB)) return;
+    const int kc = std::min(NR, it.nrhs - c0);
+    const RsWork<T> w = rs_carve<T>(it.work, it.n);
+    const T *src;
+    long lds;
+    T *dst;
+    if (kind == 0) { src = (const T *)it.B + c0; lds = it.ldb; dst = w.W1; }
+    else if (kind == 2) { src = w.W0; lds = kc; dst = w.W1; }
+    else { src = w.W1; lds = kc; dst = w.W2; }
+    trsv_body<T, NR>((const T *)it.LU, it.lda, it.n, w.rd, src, lds, dst, kc, kind, abort, q);
+}
+
+template <typename T>
+int getrs_batched_t(int nitems, const m25x_rs_item *h, const m25x_rs_item *d, int trans, int *status, hipStream_t s)
+{
+    constexpr int NR = TrsvMaxRhs<T>::NR;
+    long nmax = 0;
+    int rmax = 0, nbmax = 0;
+    for (int i = 0; i < nitems; ++i) {
+        const m25x_rs_item &it = h[i];
+        if (!it.LU || !it.perm || !it.B || !it.work || it.n < 0 || it.lda < it.n || it.nrhs < 1 ||
+            it.nrhs > MAXRHS || it.ldb < it.nrhs)
+            return fail(M25X_EINVAL, "getrs_batched: item %d: bad arguments (perm required, 1 <= nrhs <= %d)", i,
+                        MAXRHS);
+        if (it.work_bytes < getrs_work<T>(it.n, it.ldb))
+            return fail(M25X_EINVAL, "getrs_batched: item %d: workspace of %zu bytes, %zu needed", i, it.work_bytes,
+                        getrs_work<T>(it.n, it.ldb));
+        nmax = std::max(nmax, it.n);
+        rmax = std::max(rmax, it.nrhs);
+        nbmax = std::max(nbmax, (int)((it.n + SB - 1) / SB));
+    }
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, status, 1);
+    if (int rc = check_launch("getrs_batched")) return rc;
+    if (nitems == 0 || nmax == 0) return M25X_OK;
+    const long words = (long)(2 * (size_t)nmax * MAXRHS * sizeof(T) / 8);
+    const int dt = dtype_of<T>();
+    for (int c0 = 0; c0 < rmax; c0 += NR) {
+        hipLaunchKernelGGL(k_rs_prep_b<T>, dim3(blocks(std::max(words, nmax), 256), nitems), dim3(256), 0, s, d, c0,
+                           (int)(c0 == 0));
+        const dim3 mg(blocks(nmax * std::min(NR, rmax - c0), 256), nitems);
+        if (!trans) hipLaunchKernelGGL(k_rs_move_b<T>, mg, dim3(256), 0, s, d, c0, NR, 0);
+        if (int rc = check_launch("getrs_batched")) return rc;
+        const int kmax = std::min(NR, rmax - c0);
+        for (int pass = 0; pass < 2; ++pass) {
+            const int kind = trans ? pass : 2 + pass;
+            double bytes = 0, flops = 0;
+            for (int i = 0; i < nitems; ++i)
+                if (c0 < h[i].nrhs) {
+                    const double n = h[i].n, tri = kind_fwd(kind) ? 0.5 * n * (n - 1) : 0.5 * n * (n + 1);
+                    const int kc = std::min(NR, h[i].nrhs - c0);
+                    bytes += (tri + 2.0 * n * kc) * sizeof(T);
+                    flops += 8.0 * tri * kc;
+                }
+            TimedLaunch tl(s, M25X_K_TRSV, dt, bytes, flops);
+            if (kmax <= 4)
+                hipLaunchKernelGGL((k_trsv_b<T, 4>), dim3(nbmax * nitems), dim3(TB), 0, s, d, nitems, c0, kind, status);
+            else
+                hipLaunchKernelGGL((k_trsv_b<T, NR>), dim3(nbmax * nitems), dim3(TB), 0, s, d, nitems, c0, kind, status);
+            if (int rc = check_launch("getrs_batched/trsv")) return rc;
+        }
+        hipLaunchKernelGGL(k_rs_move_b<T>, mg, dim3(256), 0, s, d, c0, NR, trans ? 1 : 2);
+        if (int rc = check_launch("getrs_batched")) return rc;
+    }
+    return M25X_OK;
+}
+
 // ---- the solve without allocation ------------------------------------------------------------------------
 // The workspace of m25x_getrs_noalloc: one array per device, in the code object of this file, held until the process
 // ends. Host code never passes the array's name to a kernel or a copy (on the host it is only a stand-in): the
@@ -1065,6 +1344,27 @@ int getrs_noalloc_t(const T *LU, long n, long lda, const int *ipiv, const int *p
 }  // namespace
 
 extern "C" {
+
+int m25x_getrf_batched(int dtype, int nitems, const m25x_lu_item *items_host, const m25x_lu_item *items_dev,
+                       void *stream)
+{
+    if (nitems < 0 || (nitems && (!items_host || !items_dev))) return fail(M25X_EINVAL, "getrf_batched: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == M25X_C128) return getrf_batched_t<double2>(nitems, items_host, items_dev, s);
+    if (dtype == M25X_C64) return getrf_batched_t<float2>(nitems, items_host, items_dev, s);
+    return fail(M25X_EINVAL, "getrf_batched: dtype %d", dtype);
+}
+
+int m25x_getrs_batched(int dtype, int nitems, const m25x_rs_item *items_host, const m25x_rs_item *items_dev, int trans,
+                       int *status, void *stream)
+{
+    if (nitems < 0 || !status || (nitems && (!items_host || !items_dev)))
+        return fail(M25X_EINVAL, "getrs_batched: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == M25X_C128) return getrs_batched_t<double2>(nitems, items_host, items_dev, trans, status, s);
+    if (dtype == M25X_C64) return getrs_batched_t<float2>(nitems, items_host, items_dev, trans, status, s);
+    return fail(M25X_EINVAL, "getrs_batched: dtype %d", dtype);
+}
 
 int m25x_getrs_noalloc(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B,
                        long ldb, int nrhs, int trans, int *status, void *stream)

@@ -99,12 +99,34 @@ template <typename T> struct TNode {
     T v[TW * TW];               // their original panel rows, column-major: v[c * TW + i]
 };
 
-inline long tour_leaves(long m, int RL) { return (m + RL - 1) / RL; }
-inline long tour_nodes(long n, int RL, int AR)
+__host__ __device__ inline long tour_leaves(long m, int RL) { return (m + RL - 1) / RL; }
+__host__ __device__ inline long tour_nodes(long n, int RL, int AR)
 {
     long P = tour_leaves(n, RL), tot = 0;
     while (P > 1) { tot += P; P = (P + AR - 1) / AR; }
     return tot + 1;
+}
+
+// A tournament LU's workspace, carved from one buffer: the tree nodes, the panel's top block, the nodes' arrival
+// counts and the row moves.
+template <typename T> struct TourWork {
+    TNode<T> *nodes;
+    T *top;
+    int *arrive, *pairs;
+    long nn;
+};
+template <typename T> __host__ __device__ inline TourWork<T> tour_carve(void *work, long n)
+{
+    TourWork<T> w;
+    w.nn = tour_nodes(n, Tour<T>::RL, Tour<T>::AR);
+    char *p = (char *)work;
+    w.nodes = (TNode<T> *)p;
+    p += w.nn * sizeof(TNode<T>);
+    w.top = (T *)p;
+    p += (size_t)TW * TW * sizeof(T);
+    w.arrive = (int *)p;
+    w.pairs = w.arrive + w.nn;
+    return w;
 }
 
 // Calls f(integral_constant<int, K>) for K = B .. E - 1: the steps of an elimination with every register index a
@@ -135,6 +157,10 @@ int launch_tslu(const double2 *A, long lda, long n, long j0, int jb, int P, TNod
                 double2 *top, int *ipiv, int *pairs, int *info, hipStream_t s);
 int launch_tslu(const float2 *A, long lda, long n, long j0, int jb, int P, TNode<float2> *nodes, int *arrive,
                 float2 *top, int *ipiv, int *pairs, int *info, hipStream_t s);
+// The tournaments of panel step j0 of a batch (items[0 .. nitems)): grid (Pmax leaves, nitems); the element type
+// is the tag's (a null pointer).
+int launch_tslu_b(const double2 *tag, const m25x_lu_item *items, int nitems, long j0, int Pmax, hipStream_t s);
+int launch_tslu_b(const float2 *tag, const m25x_lu_item *items, int nitems, long j0, int Pmax, hipStream_t s);
 
 }  // namespace lu
 }  // namespace m25x

@@ -177,9 +177,8 @@ __device__ __attribute__((always_inline)) int reg_gepp(RowH<T> &x, int nr, int n
 // LAPACK's convention) and the row moves they make as (destination, source) pairs (pairs[0] = count), and info for
 // a zero pivot.
 template <typename T>
-__global__ __launch_bounds__(2 * Tour<T>::RL) void k_tslu(const T *A, long lda, long n, long j0, int jb, int P,
-                                                         TNode<T> *nodes, int *arrive, T *top, int *ipiv, int *pairs,
-                                                         int *info)
+__device__ __forceinline__ void tslu_body(const T *A, long lda, long n, long j0, int jb, int P, TNode<T> *nodes,
+                                          int *arrive, T *top, int *ipiv, int *pairs, int *info, int leaf)
 {
     constexpr int RL = Tour<T>::RL, AR = Tour<T>::AR, NT = 2 * RL, NW = NT / 64;
     __shared__ GeppLds<T, NT> gl;
@@ -189,7 +188,7 @@ __global__ __launch_bounds__(2 * Tour<T>::RL) void k_tslu(const T *A, long lda, 
     const int t = threadIdx.x, lane = t & 63;
     const int row = (t >> 6) * 32 + (lane & 31), h = lane >> 5;
     const T zero = CX<T>::make(0, 0);
-    int idx = blockIdx.x;
+    int idx = leaf;
     const long r0 = j0 + (long)idx * RL;
     int nr = (int)std::min((long)RL, n - r0);
     RowH<T> x;
@@ -329,6 +328,29 @@ __global__ __launch_bounds__(2 * Tour<T>::RL) void k_tslu(const T *A, long lda, 
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(2 * Tour<T>::RL) void k_tslu(const T *A, long lda, long n, long j0, int jb, int P,
+                                                         TNode<T> *nodes, int *arrive, T *top, int *ipiv, int *pairs,
+                                                         int *info)
+{
+    tslu_body<T>(A, lda, n, j0, jb, P, nodes, arrive, top, ipiv, pairs, info, blockIdx.x);
+}
+
+// item blockIdx.y of a batch; leaves past its own count, and items already factored, leave at once
+template <typename T>
+__global__ __launch_bounds__(2 * Tour<T>::RL) void k_tslu_b(const m25x_lu_item *items, long j0)
+{
+    const m25x_lu_item it = items[blockIdx.y];
+    const long n = it.n;
+    if (j0 >= n) return;
+    const int jb = (int)std::min((long)TW, n - j0);
+    const int P = (int)tour_leaves(n - j0, Tour<T>::RL);
+    if ((int)blockIdx.x >= P) return;
+    const TourWork<T> w = tour_carve<T>(it.work, n);
+    tslu_body<T>((const T *)it.A, it.lda, n, j0, jb, P, w.nodes, w.arrive, w.top, it.ipiv, w.pairs, it.info,
+                 blockIdx.x);
+}
+
 }  // namespace
 
 #ifdef M25X_TSLU_C64
@@ -343,6 +365,12 @@ int launch_tslu(const TT *A, long lda, long n, long j0, int jb, int P, TNode<TT>
     hipLaunchKernelGGL(k_tslu<TT>, dim3(P), dim3(2 * Tour<TT>::RL), 0, s, A, lda, n, j0, jb, P, nodes, arrive, top, ipiv,
                        pairs, info);
     return check_launch("getrf/tournament");
+}
+
+int launch_tslu_b(const TT *, const m25x_lu_item *items, int nitems, long j0, int Pmax, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tslu_b<TT>, dim3(Pmax, nitems), dim3(2 * Tour<TT>::RL), 0, s, items, j0);
+    return check_launch("getrf_batched/tournament");
 }
 
 }  // namespace lu

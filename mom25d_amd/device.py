@@ -411,6 +411,109 @@ def getrs_noalloc(F, B, trans=1, status=None):
     return B, status
 
 
+# ---- batches ------------------------------------------------------------------------------------------------
+def _item_table(cls, items):
+    """A ctypes array of items on the host and the same bytes in a device tensor (kept with the call's objects)."""
+    arr = (cls * len(items))(*items)
+    raw = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8)
+    return arr, raw.to(device(), non_blocking=False)
+
+
+class BatchFactors:
+    """The LU factors of a batch of matrices (m25x_getrf_batched): factors[i] is a Factor of Zs[i], in place."""
+
+    def __init__(self, Zs, factors, keep):
+        self.Zs, self.factors, self._keep = Zs, factors, keep
+
+
+def getrf_batched(Zs, graph=None):
+    """Tournament LUs of the square row-major device tensors Zs (one dtype, any sizes) in one sequence of launches,
+    in place; returns BatchFactors. The items go to the library largest first (its grids then shrink as the smaller
+    ones finish). graph: a torch.cuda.CUDAGraph to capture the launches into instead of running them (replay it to
+    factor; the same tensors are factored each replay)."""
+    if not Zs:
+        return BatchFactors([], [], None)
+    dt = Zs[0].dtype
+    if dt not in (c128, c64) or any(Z.dtype != dt or Z.dim() != 2 or Z.shape[0] != Z.shape[1] or Z.stride(1) != 1
+                                    for Z in Zs):
+        raise ValueError("getrf_batched: square row-major complex tensors of one dtype")
+    dv, code = device(), (L.C128 if dt == c128 else L.C64)
+    order = sorted(range(len(Zs)), key=lambda i: -Zs[i].shape[0])
+    factors, items, keep = [None] * len(Zs), [], []
+    for i in order:
+        Z = Zs[i]
+        n = Z.shape[0]
+        ipiv = torch.empty(n, dtype=i32, device=dv)
+        perm = torch.empty(n, dtype=i32, device=dv)
+        info = torch.zeros(1, dtype=i32, device=dv)
+        wb = L.lib().m25x_getrf_work_bytes(code, n)
+        work = torch.empty(max(wb, 1), dtype=torch.uint8, device=dv)
+        keep.append(work)
+        factors[i] = Factor(Z, ipiv, perm, info)
+        items.append(L.LuItem(Z.data_ptr(), n, Z.stride(0), ipiv.data_ptr(), perm.data_ptr(), info.data_ptr(),
+                              work.data_ptr(), wb))
+    host, dev_tab = _item_table(L.LuItem, items)
+    keep += [host, dev_tab]
+    if graph is not None:
+        with torch.cuda.graph(graph):
+            L.call('m25x_getrf_batched', code, len(items), host, ptr(dev_tab), stream())
+    else:
+        L.call('m25x_getrf_batched', code, len(items), host, ptr(dev_tab), stream())
+    return BatchFactors(Zs, factors, keep)
+
+
+class BatchSolve:
+    """Solves with the factors of a batch, each item's right-hand sides a column block of one (n_max, k_total)
+    tensor: the items' tables built once (m25x_getrs_batched)."""
+
+    def __init__(self, factors, cols):
+        """factors: Factor per item; cols: (first column, columns) of each item in the stacked right-hand sides."""
+        self.factors, self.cols = factors, cols
+        self.dtype = factors[0].dtype
+        self.code = L.C128 if self.dtype == c128 else L.C64
+        dv = device()
+        self.work = [torch.empty(L.lib().m25x_getrs_work_bytes(self.code, F.n, 8), dtype=torch.uint8, device=dv)
+                     for F in factors]
+        self.status = torch.zeros(1, dtype=i32, device=dv)
+
+    def __call__(self, R, trans=1, subset=None):
+        """X (n_max, k) complex128 with F_i X_i = R_i for each item's column block of R (n_max, k); subset: for
+        each item the (first column, columns) in R instead of self.cols (columns 0 skip the item)."""
+        cols = subset or self.cols
+        nmax, k = R.shape
+        X = R.to(self.dtype, copy=True).contiguous()            # solved in place: never the caller's R
+        items = []
+        for F, w, (c0, kc) in zip(self.factors, self.work, cols):
+            for p in range(0, kc, 8):
+                q = min(8, kc - p)
+                items.append(L.RsItem(F.LU.data_ptr(), F.n, F.LU.stride(0), F.perm.data_ptr(),
+                                      X.data_ptr() + (c0 + p) * X.element_size(), k, q, 0, w.data_ptr(),
+                                      w.numel()))
+        if items:
+            host, dev_tab = _item_table(L.RsItem, items)
+            L.call('m25x_getrs_batched', self.code, len(items), host, ptr(dev_tab), int(trans), ptr(self.status),
+                   stream())
+            _solve_fault(X.device).bitwise_or_(self.status)
+        return X.to(c128)
+
+
+def zgemm_nk_batched(Zs, V, cols, out=None):
+    """Y with Y_i = Z_i V_i for each item's column block (first column, columns) of V (n_max, k) complex128: one
+    launch for all items (rows past an item's n are zero in Y)."""
+    V = V.to(c128).contiguous()
+    Y = torch.zeros_like(V) if out is None else out
+    items = []
+    for Z, (c0, kc) in zip(Zs, cols):
+        for p in range(0, kc, 8):
+            q = min(8, kc - p)
+            items.append(L.MvItem(Z.data_ptr(), Z.shape[0], Z.stride(0), V.data_ptr() + (c0 + p) * 16,
+                                  Y.data_ptr() + (c0 + p) * 16, V.shape[1], Y.shape[1], q, 0))
+    if items:
+        host, dev_tab = _item_table(L.MvItem, items)
+        L.call('m25x_zgemm_nk_batched', len(items), host, ptr(dev_tab), stream())
+    return Y
+
+
 _FAULTS = {}
 
 

@@ -32,8 +32,13 @@ def _cnorm(X):
     return torch.linalg.vector_norm(X, dim=0).cpu().numpy()
 
 
-def gmres_batched(A, B, M=None, x0=None, rtol=1e-6, restart=30, maxiter=10, stall=None):
-    """(X, iterations per column) for A X = B; A, M: callables on device (N x k) complex128 tensors; B device."""
+def gmres_batched(A, B, M=None, x0=None, rtol=1e-6, restart=30, maxiter=10, stall=None, track=None,
+                  stall_each=False):
+    """(X, iterations per column) for A X = B; A, M: callables on device (N x k) complex128 tensors; B device.
+    track: called with the active columns (indices into B's columns, ascending) before A and M see a new set of
+    them, for operators that act on each column differently (a batch of systems side by side). stall_each: a
+    column that stalls stops alone (the others go on), as for independent systems; otherwise the first stall stops
+    every column, as the reference's lockstep ports do."""
     B = B.to(torch.complex128)
     N, k = B.shape
     dev = B.device
@@ -44,6 +49,8 @@ def gmres_batched(A, B, M=None, x0=None, rtol=1e-6, restart=30, maxiter=10, stal
     iters, act = np.zeros(k, int), np.arange(k)
     started = x0 is not None
     for _ in range(maxiter):
+        if track is not None:
+            track(act)
         Xa = X[:, torch.from_numpy(act).to(dev)]
         R = B[:, torch.from_numpy(act).to(dev)] - (A(Xa) if started else 0)
         beta = _cnorm(R)
@@ -53,6 +60,8 @@ def gmres_batched(A, B, M=None, x0=None, rtol=1e-6, restart=30, maxiter=10, stal
         if not len(act):
             break
         R = R[:, torch.from_numpy(np.flatnonzero(keep)).to(dev)]
+        if track is not None:
+            track(act)
         ka = len(act)
         V = torch.zeros((restart + 1, N, ka), dtype=torch.complex128, device=dev)
         Zs = torch.zeros((restart, N, ka), dtype=torch.complex128, device=dev)
@@ -116,9 +125,17 @@ def gmres_batched(A, B, M=None, x0=None, rtol=1e-6, restart=30, maxiter=10, stal
             live &= ~conv
             if not live.any():
                 break
-            if stall is not None and j + 1 >= stall[0] and (np.abs(g[j + 1])[live] > stall[1] * bn[act][live]).any():
-                last[live], stalled = j, True
-                break
+            if stall is not None and j + 1 >= stall[0]:
+                bad = live & (np.abs(g[j + 1]) > stall[1] * bn[act])
+                if stall_each:
+                    last[bad] = j
+                    live &= ~bad
+                    stalled = stalled or bool(bad.any())
+                    if not live.any():
+                        break
+                elif bad.any():
+                    last[live], stalled = j, True
+                    break
         for c in range(ka):
             m = last[c] + 1
             if not m:

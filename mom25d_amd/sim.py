@@ -415,7 +415,11 @@ class Simulation:
 
     def sparams(self, f, keep=False):
         """S at f for the lumped ports (sim.py:1045-1092)."""
-        I, n = self.currents(f, self.excitation()), len(self.ports)
+        return self.sparams_from(f, self.currents(f, self.excitation()), keep)
+
+    def sparams_from(self, f, I, keep=False):
+        """S at f from the basis currents I of the excitation() columns."""
+        n = len(self.ports)
         Ap, Bp = np.zeros((n, n), complex), np.zeros((n, n), complex)
         for i, p in enumerate(self.ports):
             Vp, Ip = np.eye(n)[i], self.port_current(I, i)
