@@ -1,5 +1,5 @@
 """Frequency-point solves/s on one MI355X: the seeded 4-layer board (N ~ 7.8k unknowns, 6 vias, 4 via ports)
-swept over 16 frequencies geomspace(1, 8 GHz), mixed precision (complex64 LU, its trailing updates accumulated on the FP64 matrix cores, FP64 GMRES
+swept over 16 frequencies geomspace(1, 8 GHz), mixed precision (complex64 LU, its trailing updates on the FP32 matrix cores with FP32 accumulation, FP64 GMRES
 refinement, factors kept and reused across frequencies) and, beside it, the plain FP64 path (complex128 LU for
 every point).
 
@@ -55,9 +55,10 @@ def sweep(sim, freqs, steps):
 
 PEAK = {'hbm': (HBM_PEAK, 'GB/s', 1e9), 'fp64': (FP64_PEAK, 'TFLOP/s', 1e12), 'fp32': (FP32_PEAK, 'TFLOP/s', 1e12)}
 # the roofline that bounds each kernel: HBM for the streaming ones, the matrix cores the trailing update runs on
-# (FP64 for both element types of the tournament LU: its complex64 update accumulates in FP64)
+# (complex128: FP64; complex64: FP32 with FP32 accumulation, or FP64 with --update fp64)
 BOUND = dict(fill_far='hbm', near_scatter='hbm', apply_moves='hbm', trsv='hbm', zgemm_nk='hbm', cgs2='hbm',
              laswp='hbm', gemm='fp64', near_moments='fp64', tslu='fp64', l21='fp64', trsm='fp64', panel='fp64')
+UPDATE = {'c64': 'fp32'}
 
 
 def kernel_shares(rec):
@@ -75,6 +76,8 @@ def roofline_of(rec, kernel, dtype):
     sel = (rec['kernel'] == kernel) & (rec['dtype'] == dtype)
     ms = rec['ms'][sel]
     bound = BOUND[kernel]
+    if kernel == 'gemm' and dtype == 'c64':
+        bound = UPDATE['c64']
     peak, unit, scale = PEAK[bound]
     work = rec['bytes'][sel] if bound == 'hbm' else rec['flops'][sel]
     achieved = float(work.sum() / (ms.sum() * 1e-3)) if ms.sum() > 0 else 0.0
@@ -210,12 +213,184 @@ def note(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
+def batch_points(config):
+    """The (simulation, frequency) points of a batch configuration and its description. c3: the seeded 2-layer
+    coupled-line board (seed 3) at 64 frequencies geomspace(0.5, 10 GHz); c4: 32 one-layer trace boards
+    (boards.uneven_board, seeds 0-31: 2-16 traces each, meshed to N ~ U(500, 6000)) at 8 frequencies U(1, 8 GHz) each
+    (rng seed 1000 + board seed)."""
+    from mom25d_amd import boards
+    from mom25d_amd.sim import Simulation
+
+    def make(board, *args):
+        st, sh, po, vi, kw = board(*args)
+        return Simulation(st, sh, po, vi, precision='mixed', verbose=False, **kw)
+    if config == 'c3':
+        sim = make(boards.coupled_lines)
+        freqs = np.geomspace(0.5e9, 10e9, 64)
+        N = sim.Ne + sim.Nv
+        return [(sim, f) for f in freqs], dict(
+            workload=f"C3: seeded 2-layer coupled-line board (seed 3), N={N}, 64 frequencies geomspace(0.5, 10 GHz) "
+                     f"as one batch, mixed precision", N=int(N), board_seed=3, points=64)
+    sims, pts = [], []
+    for seed in range(32):
+        st, sh, po, vi, kw = boards.uneven_board(seed)
+        kw.pop('target_n')
+        sim = Simulation(st, sh, po, vi, precision='mixed', verbose=False, **kw)
+        sims.append(sim)
+        fr = np.random.default_rng(1000 + seed).uniform(1e9, 8e9, 8)
+        pts += [(sim, f) for f in fr]
+    Ns = [s.Ne + s.Nv for s in sims]
+    return pts, dict(workload=f"C4: 32 seeded one-layer trace boards (seeds 0-31, 2-16 traces, meshed to N ~ U(500, 6000): "
+                              f"{min(Ns)}-{max(Ns)}) x 8 "
+                              f"frequencies U(1, 8 GHz) as one batch of 256 points, mixed precision",
+                     N_min=int(min(Ns)), N_max=int(max(Ns)), N_mean=round(float(np.mean(Ns)), 1), points=len(pts))
+
+
+def batch_bench(config, a):
+    """Solves/s of a whole batch (the points filled on the device, factored by the batched LU - tile depth chosen
+    by the autotuner - and refined by one GMRES over every point), against the same points solved one at a time
+    by the single-point mixed path (with its factor reuse), the batched factor's rate and its HIP-graph replay."""
+    import torch
+    from mom25d_amd import device as dv, tune
+    from mom25d_amd.batch import SweepBatch
+    pts, cfg = batch_points(config)
+    note(f"{config}: {len(pts)} points built")
+    Ns = sorted({s.Ne + s.Nv for s, _ in pts})
+    probe = [int(n) for n in np.quantile(Ns, np.linspace(0, 1, min(8, len(Ns)))).astype(int)]
+    rec = tune.tune_lu(probe)
+    note(f"autotuner: K-stage {rec['choice']} ({rec['ms']} ms on sizes {probe})")
+    SweepBatch(pts[:4]).run()                                   # warm-up (kernels loaded, tables cached)
+    torch.cuda.synchronize()
+    b = SweepBatch(pts)
+    t0 = time.perf_counter()
+    S = b.run()
+    torch.cuda.synchronize()
+    T = time.perf_counter() - t0
+    st = b.stats
+    note(f"batch: {len(pts)} points in {T:.3f} s ({st})")
+    # the factor stage again, eager and as a replayed HIP graph, on copies of the points' complex64 matrices
+    flops = sum(8.0 * (s.Ne + s.Nv) ** 3 / 3 for s, _ in pts)
+    Z64s = []
+    for sim, f in pts[:min(len(pts), 64)]:
+        Z = sim.fast.fill(f)
+        Z64 = Z.to(dv.c64)
+        del Z
+        Z64s.append(Z64)
+    src = [z.clone() for z in Z64s]
+    sub_flops = sum(8.0 * z.shape[0] ** 3 / 3 for z in Z64s)
+    s_ = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s_)
+    dv.getrf_batched(Z64s)
+    e1.record(s_)
+    torch.cuda.synchronize()
+    eager_ms = e0.elapsed_time(e1)
+    g = torch.cuda.CUDAGraph()
+    work = [z.clone() for z in src]
+    dv.getrf_batched(work, graph=g)
+    reps = []
+    for _ in range(3):
+        for w, z in zip(work, src):
+            w.copy_(z)
+        e0.record(s_)
+        g.replay()
+        e1.record(s_)
+        torch.cuda.synchronize()
+        reps.append(e0.elapsed_time(e1))
+    del work, src, Z64s, g
+    # the same points one at a time through the single-point mixed path (its factor reuse on)
+    seq = len(pts)
+    for sim in {id(s): s for s, _ in pts}.values():
+        sim.fast._kept, sim.fast._mixed_failed = [], 0.0
+    t0 = time.perf_counter()
+    Sseq = [sim.sparams(f) for sim, f in pts[:seq]]
+    torch.cuda.synchronize()
+    Tseq = time.perf_counter() - t0
+    note(f"one at a time: {seq} points in {Tseq:.3f} s")
+    dS = max(float(np.abs(S[i] - Sseq[i]).max()) for i in range(seq))
+    value = len(pts) / T
+    out = dict(metric="freq-point solves/s (batched: fill + factor + refinement of all points)",
+               value=round(value, 3), unit="solves/s", n_gpus=1, steps=len(pts), warmup=4,
+               ms_per_step=round(1e3 * T / len(pts), 3), higher_is_better=True, scaling="weak", vs_baseline=None,
+               dtype="complex64 LU + complex128 fill/refinement", data="synthetic",
+               batch_s=round(T, 4), stages_s={k: round(v, 4) for k, v in st.items() if k.endswith('_s')},
+               fallback_points=st['fallback'],
+               one_at_a_time=dict(value=round(seq / Tseq, 3), points=seq, ms_per_point=round(1e3 * Tseq / seq, 3)),
+               batch_over_one_at_a_time=round(value / (seq / Tseq), 3),
+               max_dS_batch_vs_one_at_a_time=dS,
+               factor=dict(tflops_in_batch=round(flops / st['factor_s'] / 1e12, 2),
+                           eager_ms=round(eager_ms, 3), graph_replay_ms=[round(x, 3) for x in reps],
+                           points_timed=min(len(pts), 64), tflops_eager=round(sub_flops / (eager_ms * 1e-3) / 1e12, 2)),
+               autotune=rec, config=cfg)
+    print(json.dumps(out))
+
+
+def serve_bench(a):
+    """C5: Poisson streams of frequency-point requests over 16 resident boards (boards.uneven_board, seeds 100-115,
+    meshed to N ~ U(2000, 20000)), each board's frequencies in the order an adaptive sweep proposes them; for each
+    rate, 32 requests through the pipelined batching server (memory admission against 0.85 x the free HBM): p50 /
+    p99 latency, solves/s, mean batch; the top rate's stream again one request at a time (no batching, no
+    pipelining) for comparison."""
+    import torch
+    from mom25d_amd import boards
+    from mom25d_amd.serve import Server, adaptive_order, request_bytes, summarize
+    from mom25d_amd.sim import Simulation
+    sims = {}
+    for seed in range(100, 116):
+        st, sh, po, vi, kw = boards.uneven_board(seed, n_range=(2000, 20000))
+        kw.pop('target_n')
+        sims[f'b{seed}'] = Simulation(st, sh, po, vi, precision='mixed', verbose=False, **kw)
+    Ns = {k: s.Ne + s.Nv for k, s in sims.items()}
+    note(f"c5: 16 boards resident, N {min(Ns.values())}-{max(Ns.values())}")
+    names = sorted(sims)
+    srv = Server(sims)
+    srv.serve([(0.0, names[0], 2e9), (0.0, names[1], 2e9)])                 # warm-up
+    rates, per = (5, 10, 20, 50, 100, 200), 32
+    order = {b: iter(adaptive_order(1e9, 8e9, 64)) for b in names}
+    rng = np.random.default_rng(5)
+    out_rates = {}
+    last = None
+    for lam in rates:
+        t, reqs = 0.0, []
+        for _ in range(per):
+            t += rng.exponential(1.0 / lam)
+            b = names[int(rng.integers(len(names)))]
+            reqs.append((t, b, next(order[b])))
+        recs = srv.serve(reqs)
+        sm = summarize(recs)
+        out_rates[str(lam)] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in sm.items()}
+        note(f"rate {lam}/s: {sm}")
+        last = reqs
+    one = summarize(Server(sims, max_batch=1).serve(last, pipeline=False))
+    note(f"one at a time at {rates[-1]}/s: {one}")
+    top = out_rates[str(rates[-1])]
+    out = dict(metric="freq-point solves/s under a Poisson request stream (16 resident boards, N up to 20k); "
+                      "p50/p99 solve latency",
+               value=top['solves_per_s'], unit="solves/s", n_gpus=1, steps=per * len(rates), warmup=2,
+               ms_per_step=round(1e3 / top['solves_per_s'], 3), higher_is_better=True, scaling="weak",
+               vs_baseline=None, dtype="complex64 LU + complex128 fill/refinement", data="synthetic",
+               p50_ms=round(1e3 * top['p50_s'], 3), p99_ms=round(1e3 * top['p99_s'], 3), rates=out_rates,
+               one_at_a_time_top_rate={k: (round(v, 4) if isinstance(v, float) else v) for k, v in one.items()},
+               budget_gb=round(srv.budget / 1e9, 1),
+               config=dict(workload=f"C5: 16 resident seeded boards (seeds 100-115, N {min(Ns.values())}-"
+                                    f"{max(Ns.values())}), Poisson arrivals at {list(rates)} req/s, {per} requests "
+                                    f"each, adaptive-sweep frequencies in 1-8 GHz, mixed precision",
+                           N=Ns, largest_request_gb=round(max(request_bytes(s) for s in sims.values()) / 1e9, 2)))
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=16)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--update', default='fp32', choices=('fp64', 'fp32'),
+                    help="the complex64 LU's trailing update on the FP32 (default: FP32 accumulation) or FP64 matrix "
+                         "cores")
+    ap.add_argument('--config', default='c2', choices=('c2', 'c3', 'c4', 'c5'),
+                    help="c2: the seeded 4-layer board's sweep (the headline metric); c3 / c4: the batched sweeps; "
+                         "c5: the serving stream")
     ap.add_argument('--no-double', action='store_true')
     ap.add_argument('--roofline-only', action='store_true',
                     help="only the roofline kernels, for profilers: the complex64 LU of the board at 2 GHz, ten "
@@ -227,6 +402,15 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    from mom25d_amd import _lib
+    _lib.call('m25x_lu_set_update_precision', int(a.update == 'fp32'))
+    UPDATE['c64'] = a.update
+    if a.config in ('c3', 'c4'):
+        batch_bench(a.config, a)
+        return
+    if a.config == 'c5':
+        serve_bench(a)
+        return
     freqs = np.geomspace(1e9, 8e9, 16)
     sm = build_sim('mixed')
     N = sm.Ne + sm.Nv
@@ -291,9 +475,11 @@ def main():
         del sd
     lu = lu_rate(sm, freqs[5])
     note("LU rates done")
-    out['lu'] = {k: dict(ms=round(v['ms'], 3), tflops=round(v['tflops'], 2), peak_tflops=FP64_PEAK / 1e12,
-                         frac=round(v['tflops'] * 1e12 / FP64_PEAK, 4),
-                         matrix_cores='FP64 (complex64: FP64 accumulation, one rounding to FP32)' if k == 'c64'
+    pk = lambda k: FP32_PEAK if (k == 'c64' and a.update == 'fp32') else FP64_PEAK
+    out['lu'] = {k: dict(ms=round(v['ms'], 3), tflops=round(v['tflops'], 2), peak_tflops=pk(k) / 1e12,
+                         frac=round(v['tflops'] * 1e12 / pk(k), 4),
+                         matrix_cores=('FP32 (complex64: FP32 accumulation)' if a.update == 'fp32' else
+                                       'FP64 (complex64: FP64 accumulation, one rounding to FP32)') if k == 'c64'
                          else 'FP64')
                  for k, v in lu.items()}
     rff = kernel_roofline(sm, freqs[5])

@@ -249,6 +249,19 @@ int m25x_getrs(int dtype, const void *LU, long n, long lda, const int *ipiv, con
 int m25x_getrs_noalloc(int dtype, const void *LU, long n, long lda, const int *ipiv, const int *perm, void *B,
                        long ldb, int nrhs, int trans, int *status, void *stream);
 
+/* The K-stage depth of the tournament LU's trailing update on the matrix cores (8, 16 or 32 columns of L21 and rows
+ * of U12 staged in LDS between two barriers), for every later m25x_getrf / m25x_getrf_batched in the process (8 until
+ * set); the results do not depend on it except by rounding order within a stage's sums. No counterpart in the
+ * reference: the knob of this build's autotuner (mom25d_amd/tune.py), which times the choices on the device. */
+int m25x_lu_set_gemm_stage(int ks);
+int m25x_lu_gemm_stage(void);
+
+/* The matrix cores of the complex64 tournament LU's trailing update, for every later call in the process: 1 (the
+ * default) FP32 with FP32 accumulation (twice the FP64 peak rate), 0 FP64 (products and sums in FP64, one rounding
+ * to FP32 an update: a smaller backward error). The GPU backend's complex64 factor (src/mom25d/gpu.py:69-74) runs
+ * complex FP32 throughout; this selects how much of it. */
+int m25x_lu_set_update_precision(int fp32);
+
 /* ---- batches: many systems in one launch per step ------------------------------------------------------
  * No counterpart in the reference, whose solver takes one frequency at a time (src/mom25d/run.py:55-64 calls
  * Simulation.sparams once a point; src/mom25d/fast.py:704-850 factors one Z). These run the same kernels as

@@ -86,6 +86,9 @@ _SIGS = {
     'm25x_getrs_work_bytes': ([I, L, L], S),
     'm25x_getrs': ([I, P, L, L, P, P, P, L, I, I, P, S, P], I),
     'm25x_getrs_noalloc': ([I, P, L, L, P, P, P, L, I, I, P, P], I),
+    'm25x_lu_set_gemm_stage': ([I], I),
+    'm25x_lu_gemm_stage': ([], I),
+    'm25x_lu_set_update_precision': ([I], I),
     'm25x_getrf_batched': ([I, I, P, P, P], I),
     'm25x_getrs_batched': ([I, I, P, P, I, P, P], I),
     'm25x_zgemm_nk_batched': ([I, P, P, P], I),

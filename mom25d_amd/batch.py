@@ -7,7 +7,8 @@ Simulation.sparams per frequency; src/mom25d/fast.py:704-850 fills and factors o
   factor    all complex64 copies in one sequence of launches (m25x_getrf_batched: each panel step one launch per
             kernel for all points), optionally captured once in a HIP graph and replayed for the next batch of the
             same sizes
-  refine    one GMRES over every point's ports side by side (columns of one (n_max, k_total) block, zero-padded):
+  refine    one GMRES over the ports of every point of similar size (groups within GROUP_SPREAD), side by side as
+            the columns of one (n_max, k_total) block, zero-padded:
             the operator is block-diagonal - the FP64 products of all points in one launch (m25x_zgemm_nk_batched),
             the preconditioner all points' triangular solves in one launch per triangle (m25x_getrs_batched) - and
             a point whose refinement stalls stops alone
@@ -25,6 +26,8 @@ import torch
 from . import device as dv
 from .fast import FRESH_STALL, MIXED_MAX_ITER, POLISH
 from .linalg import gmres_batched
+
+GROUP_SPREAD = 1.25      # points refined together: sizes within this factor of the group's smallest
 
 
 class _Cols:
@@ -59,6 +62,21 @@ class SweepBatch:
 
     def run(self):
         """S of every point (a list of nports x nports arrays, in the order given)."""
+        self.fill_factor()
+        return self.refine()
+
+    def tensors(self):
+        """The device tensors the first stage leaves for the second (for a caller that runs the stages on two
+        streams: record them on the second)."""
+        st = self._state
+        out = st['Zs'] + st['Z64s'] + st['ds'] + st['Bs']
+        for f in st['F'].factors:
+            out += [f.LU, f.ipiv, f.perm, f.info]
+        return out + [k for k in st['F']._keep if isinstance(k, torch.Tensor)]
+
+    def fill_factor(self):
+        """Stage one, queued on the current stream: every point's Z filled on the device and equilibrated, its
+        complex64 copy, and all copies factored by the batched LU."""
         t0 = time.perf_counter()
         pts = self.points
         Zs, Z64s, ds, Bs, ks = [], [], [], [], []
@@ -69,7 +87,6 @@ class SweepBatch:
             dv.scale_c64(Z, d, Z64)                          # Z <- D Z D (FP64, kept), Z64 its complex64 copy
             E = torch.from_numpy(sim.excitation()).to(Z.device)
             Zs.append(Z); Z64s.append(Z64); ds.append(d); Bs.append(d[:, None] * E); ks.append(E.shape[1])
-        torch.cuda.synchronize()
         t1 = time.perf_counter()
         if self.graph:
             g = torch.cuda.CUDAGraph()
@@ -78,16 +95,40 @@ class SweepBatch:
             self._graph = (g, F)
         else:
             F = dv.getrf_batched(Z64s)
-        torch.cuda.synchronize()
+        self._state = dict(Zs=Zs, Z64s=Z64s, ds=ds, Bs=Bs, ks=ks, F=F, t0=t0, t1=t1)
+
+    def refine(self):
+        """Stage two, on the current stream (after the first stage's work): the refinement in size groups and S per
+        point, a point that does not converge solved again by its board's single-point path."""
+        st = self._state
+        Zs, ds, Bs, ks, F, t0, t1 = st['Zs'], st['ds'], st['Bs'], st['ks'], st['F'], st['t0'], st['t1']
+        pts = self.points
+        torch.cuda.current_stream().synchronize()
         t2 = time.perf_counter()
-        Y, ok = self._refine(Zs, F.factors, Bs, ks)
-        torch.cuda.synchronize()
+        # one GMRES per group of points of similar size (the stacked block is padded to the group's largest n)
+        order = np.argsort([Z.shape[0] for Z in Zs], kind='stable')
+        groups, cur = [], [order[0]]
+        for i in order[1:]:
+            if Zs[i].shape[0] > GROUP_SPREAD * Zs[cur[0]].shape[0]:
+                groups.append(cur)
+                cur = []
+            cur.append(i)
+        groups.append(cur)
+        I_of, ok, self.iters = {}, {}, [None] * len(pts)
+        for grp in groups:
+            Y, okg, it = self._refine([Zs[i] for i in grp], [F.factors[i] for i in grp], [Bs[i] for i in grp],
+                                      [ks[i] for i in grp])
+            first = np.concatenate([[0], np.cumsum([ks[i] for i in grp])[:-1]])
+            for j, i in enumerate(grp):
+                ok[i] = okg[j]
+                I_of[i] = ds[i][:, None] * Y[:Zs[i].shape[0], first[j]:first[j] + ks[i]]
+                self.iters[i] = it[first[j]:first[j] + ks[i]]
+        torch.cuda.current_stream().synchronize()
         t3 = time.perf_counter()
         out, fallback = [], 0
         for i, (sim, f) in enumerate(pts):
             if ok[i]:
-                c0, k = int(self.cols.first[i]), ks[i]
-                I = (ds[i][:, None] * Y[:Zs[i].shape[0], c0:c0 + k]).cpu().numpy()
+                I = I_of[i].cpu().numpy()
             else:
                 fallback += 1
                 I = sim.fast.solve_direct(f, sim.excitation(), precision='double')
@@ -96,13 +137,16 @@ class SweepBatch:
             raise RuntimeError("a batched triangular solve ran out of a wait bound: its results are not solutions")
         t4 = time.perf_counter()
         self.stats = dict(points=len(pts), fill_s=t1 - t0, factor_s=t2 - t1, refine_s=t3 - t2, s_s=t4 - t3,
-                          total_s=t4 - t0, fallback=fallback, iterations=self.iters)
+                          total_s=t4 - t0, fallback=fallback, groups=len(groups),
+                          iterations_max=int(max(max(x) for x in self.iters)))
+        self._state = None
         return out
 
     def _refine(self, Zs, factors, Bs, ks):
-        """GMRES over all points' columns side by side: (Y (n_max, k_total), converged per point)."""
+        """GMRES over the given points' columns side by side: (Y (n_max, k_total), converged per point, iterations
+        per column)."""
         nmax = max(Z.shape[0] for Z in Zs)
-        self.cols = cols = _Cols(ks)
+        cols = _Cols(ks)
         B = torch.zeros((nmax, int(cols.k.sum())), dtype=dv.c128, device=Zs[0].device)
         for i, b in enumerate(Bs):
             B[:b.shape[0], cols.first[i]:cols.first[i] + ks[i]] = b
@@ -131,6 +175,5 @@ class SweepBatch:
             Y[:, sub] = Y2
             for c, n in zip(more, it2):
                 it[c] += n
-        self.iters = it
         ok = [bool(okc[cols.first[i]:cols.first[i] + ks[i]].all()) for i in range(len(ks))]
-        return Y, ok
+        return Y, ok, it

@@ -6,7 +6,9 @@
                         Manhattan traces (width U(0.15, 0.5) mm) on interfaces 2 and 4, 6 vias (r 0.15 mm) joining
                         them, 4 via ports to ground
   coupled_lines(seed)   C3: a 2-layer board with two parallel 0.3 mm strips 0.2 mm apart, 40 mm long, 4 ports
-  trace_board(seed)     C4 / C5: a one-layer board of rng.integers(2, 17) traces from the C1 generator
+  trace_board(seed)     C5: a one-layer board of rng.integers(2, 17) traces from the C1 generator
+  uneven_board(seed)    C4: the same kind of board (narrower, shorter traces) meshed to a drawn size N ~ U(500, 6000)
+                        (the mesh density, and where needed the cells across a trace, solved for)
 Each returns (stack, shapes, ports, vias, kwargs) for Simulation(stack, shapes, ports, vias, **kwargs); the seeds
 and the drawn parameters are deterministic (numpy default_rng).
 """
@@ -98,3 +100,36 @@ def trace_board(seed, dmax=0.3 * MM, ncross=4, ntrace=None):
             ports.append(ViaPort(Via(float(start[0]), float(start[1]), 0, 1, 0.15 * MM, name=f'P{i + 1}'), 50.0,
                                  f'P{i + 1}'))
     return st, shapes, ports, [], dict(dmax=dmax, ncross=ncross, w_ref={1: 1.5 * MM})
+
+
+def uneven_board(seed, n_range=(500, 6000), ncross=3):
+    """A one-layer FR4 board (the C1 stack) of rng.integers(2, 17) traces of width U(0.3, 0.6) mm and 1-3 segments
+    of U(1, 2.5) mm, a via port to ground at the first four traces' starts, meshed to a size drawn from
+    U(n_range): the mesh density dmax is solved for (the unknowns grow as 1 / dmax along the traces), between 0.02
+    and 2 mm, with fewer cells across the traces where even the coarsest mesh has too many unknowns. Returns the usual (stack, shapes, ports, vias, kwargs) with kwargs['target_n'] the drawn size."""
+    from .sim import Simulation
+    rng = np.random.default_rng(seed)
+    st = LayerStack([Layer(0.8 * MM, 4.4, 0.02)])
+    n = int(rng.integers(2, 17))
+    shapes, ports = {1: []}, []
+    for i in range(n):
+        w = _snap(rng.uniform(0.3, 0.6) * MM, 0.01 * MM)
+        start = _snap(rng.uniform(-12, 12, 2) * MM)
+        r, _ = _trace(rng, start, w, nseg=int(rng.integers(1, 4)), seg=(1.0 * MM, 2.5 * MM), box=15 * MM)
+        shapes[1] += [tuple(_snap(x, 0.01 * MM)) for x in r]
+        if i < 4:
+            ports.append(ViaPort(Via(float(start[0]), float(start[1]), 0, 1, 0.15 * MM, name=f'P{i + 1}'), 50.0,
+                                 f'P{i + 1}'))
+    target = float(rng.uniform(*n_range))
+
+    def size(dmax, nc):
+        sim = Simulation(st, shapes, ports, [], verbose=False, mesh_only=True, dmax=dmax, ncross=nc,
+                         w_ref={1: 0.6 * MM})
+        return sim.Ne + sim.Nv
+    for nc in range(ncross, 0, -1):             # fewer cells across the traces where even coarse ones are too many
+        dmax = 0.3 * MM
+        for _ in range(5):
+            dmax = float(np.clip(dmax * size(dmax, nc) / target, 0.02 * MM, 2.0 * MM))
+        if size(dmax, nc) <= 1.25 * target:
+            break
+    return st, shapes, ports, [], dict(dmax=dmax, ncross=nc, w_ref={1: 0.6 * MM}, target_n=int(target))

@@ -19,6 +19,7 @@
 // The solves (getrs): one launch per triangle, one workgroup per 64-row diagonal block; each accumulates the
 // products of its rows with the earlier blocks as their values appear, then solves its own block and hands its
 // values on as self-marking words.
+#include <atomic>
 #include <mutex>
 #include "m25x_lu_dev.h"
 
@@ -289,15 +290,15 @@ __global__ __launch_bounds__(256) void k_trsm(T *A, long lda, long n, long j0, i
 // C's columns), its second A, so a lane's outputs run down C's columns (coalesced stores).
 constexpr int GM = 64, GN = 64, GK = 8;
 
-template <typename T, bool SUB, typename AC = T>
+template <typename T, bool SUB, typename AC = T, int KS = GK>
 __device__ __forceinline__ void gemm_body(T *C, long ldc, const T *A, long lda, const T *B, long ldb, long M, long N,
                                           long K, long bx, long by)
 {
     typedef typename CX<AC>::R R;           // the matrix cores' type: products and sums in it, one rounding to T
     typedef typename CX<T>::R TR;
     typedef typename CX<AC>::acc4 acc4;
-    __shared__ T As[GK][GM];         // As[p][i] = A[i][p]
-    __shared__ T Bs[GK][GN + 1];     // Bs[p][j] = B[p][j]
+    __shared__ T As[KS][GM];         // As[p][i] = A[i][p]
+    __shared__ T Bs[KS][GN + 1];     // Bs[p][j] = B[p][j]
     const long m0 = bx * GM, n0 = by * GN;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;     // wave tile origin in the workgroup tile
@@ -309,21 +310,21 @@ __device__ __forceinline__ void gemm_body(T *C, long ldc, const T *A, long lda, 
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) { cr[a][b][r] = 0; ci[a][b][r] = 0; }
-    for (long k0 = 0; k0 < K; k0 += GK) {
-        // stage A (GM x GK) and B (GK x GN); zero outside
-        for (int idx = tid; idx < GM * GK; idx += 256) {
+    for (long k0 = 0; k0 < K; k0 += KS) {
+        // stage A (GM x KS) and B (KS x GN); zero outside
+        for (int idx = tid; idx < GM * KS; idx += 256) {
             const int i = idx % GM, p = idx / GM;
             const long gi = m0 + i, gp = k0 + p;
             As[p][i] = ldz(gi < M && gp < K, A + gp * lda + gi);
         }
-        for (int idx = tid; idx < GK * GN; idx += 256) {
-            const int p = idx % GK, j = idx / GK;
+        for (int idx = tid; idx < KS * GN; idx += 256) {
+            const int p = idx % KS, j = idx / KS;
             const long gp = k0 + p, gj = n0 + j;
             Bs[p][j] = ldz(gp < K && gj < N, B + gj * ldb + gp);
         }
         __syncthreads();
 #pragma unroll
-        for (int ks = 0; ks < GK / 2; ++ks) {
+        for (int ks = 0; ks < KS / 2; ++ks) {
             const int p = 2 * ks + ph;
             R av[2], br[2], bi[2];
 #pragma unroll
@@ -367,11 +368,44 @@ __device__ __forceinline__ void gemm_body(T *C, long ldc, const T *A, long lda, 
             }
 }
 
-template <typename T, bool SUB, typename AC = T>
+template <typename T, bool SUB, typename AC = T, int KS = GK>
 __global__ __launch_bounds__(256) void k_gemm(T *C, long ldc, const T *A, long lda, const T *B, long ldb, long M,
                                               long N, long K)
 {
-    gemm_body<T, SUB, AC>(C, ldc, A, lda, B, ldb, M, N, K, blockIdx.x, blockIdx.y);
+    gemm_body<T, SUB, AC, KS>(C, ldc, A, lda, B, ldb, M, N, K, blockIdx.x, blockIdx.y);
+}
+
+// The depth of the K stages of the tournament LU's trailing update (8, 16 or 32: the number of columns of L21 and
+// rows of U12 staged in LDS between two barriers), chosen at run time by m25x_lu_set_gemm_stage (an autotuner
+// times the choices); 8 until set.
+std::atomic<int> g_gemm_stage{8};
+// The complex64 trailing update's matrix cores: 1 FP32 (the default: v_mfma_f32_16x16x4f32, FP32 accumulation, twice
+// the FP64 peak rate), 0 FP64 (v_mfma_f64_16x16x4f64, products and sums in FP64, one rounding to FP32 per update:
+// a smaller backward error); m25x_lu_set_update_precision. complex128 always updates in FP64.
+std::atomic<int> g_update_fp32{1};
+
+template <typename T, typename AC>
+void launch_update_ac(hipStream_t s, dim3 grid, T *C, long ldc, const T *A, long lda, const T *B, long ldb, long M,
+                      long N, long K)
+{
+    switch (g_gemm_stage.load(std::memory_order_relaxed)) {
+    case 16: hipLaunchKernelGGL((k_gemm<T, true, AC, 16>), grid, dim3(256), 0, s, C, ldc, A, lda, B, ldb, M, N, K); break;
+    case 32: hipLaunchKernelGGL((k_gemm<T, true, AC, 32>), grid, dim3(256), 0, s, C, ldc, A, lda, B, ldb, M, N, K); break;
+    default: hipLaunchKernelGGL((k_gemm<T, true, AC, 8>), grid, dim3(256), 0, s, C, ldc, A, lda, B, ldb, M, N, K);
+    }
+}
+
+template <typename T>
+void launch_update(hipStream_t s, dim3 grid, T *C, long ldc, const T *A, long lda, const T *B, long ldb, long M,
+                   long N, long K)
+{
+    if constexpr (sizeof(T) == 8) {
+        if (g_update_fp32.load(std::memory_order_relaxed)) {
+            launch_update_ac<T, float2>(s, grid, C, ldc, A, lda, B, ldb, M, N, K);
+            return;
+        }
+    }
+    launch_update_ac<T, double2>(s, grid, C, ldc, A, lda, B, ldb, M, N, K);
 }
 
 // ---- the interchanges as a permutation: perm = the swaps of ipiv applied in order to 0..n-1 ------------------
@@ -953,9 +987,8 @@ int getrf_tp(T *A, long n, long lda, int *ipiv, int *perm, int *info, void *work
             }
             const long M = n - c0;
             TimedLaunch tl(s, M25X_K_GEMM, dt, es * (2.0 * M * M + 2.0 * M * jb), 8.0 * M * M * jb);
-            hipLaunchKernelGGL((k_gemm<T, true, double2>), dim3(blocks(M, GM), blocks(M, GN)), dim3(256), 0, s,
-                               A + c0 * lda + c0, lda, (const T *)(A + j0 * lda + c0), lda, (const T *)A12, lda, M, M,
-                               (long)jb);
+            launch_update<T>(s, dim3(blocks(M, GM), blocks(M, GN)), A + c0 * lda + c0, lda, (const T *)(A + j0 * lda + c0),
+                             lda, (const T *)A12, lda, M, M, (long)jb);
         }
         if (int rc = check_launch("getrf/update")) return rc;
     }
@@ -1104,7 +1137,7 @@ __global__ __launch_bounds__(256) void k_trsm_b(const m25x_lu_item *items, long 
     trsm_body<T, TW>((T *)it.A, it.lda, n, j0, jb, c0, blockIdx.x);
 }
 
-template <typename T>
+template <typename T, int KS, typename AC = double2>
 __global__ __launch_bounds__(256) void k_gemm_b(const m25x_lu_item *items, long j0)
 {
     const m25x_lu_item it = items[blockIdx.z];
@@ -1115,8 +1148,8 @@ __global__ __launch_bounds__(256) void k_gemm_b(const m25x_lu_item *items, long 
     if (M <= 0 || (long)blockIdx.x * GM >= M || (long)blockIdx.y * GN >= M) return;
     T *A = (T *)it.A;
     const long lda = it.lda;
-    gemm_body<T, true, double2>(A + c0 * lda + c0, lda, A + j0 * lda + c0, lda, A + c0 * lda + j0, lda, M, M, jb,
-                                blockIdx.x, blockIdx.y);
+    gemm_body<T, true, AC, KS>(A + c0 * lda + c0, lda, A + j0 * lda + c0, lda, A + c0 * lda + j0, lda, M, M, jb,
+                               blockIdx.x, blockIdx.y);
 }
 
 template <typename T>
@@ -1164,8 +1197,21 @@ int getrf_batched_t(int nitems, const m25x_lu_item *h, const m25x_lu_item *d, hi
         if (mbig > TW) {
             hipLaunchKernelGGL((k_trsm_b<T>), dim3(blocks(mbig - TW, TC), na), dim3(256), 0, s, d, j0);
             TimedLaunch tl(s, M25X_K_GEMM, dt, 0.0, flops);
-            hipLaunchKernelGGL(k_gemm_b<T>, dim3(blocks(mbig - TW, GM), blocks(mbig - TW, GN), na), dim3(256), 0, s, d,
-                               j0);
+            const dim3 grid(blocks(mbig - TW, GM), blocks(mbig - TW, GN), na);
+            const bool f32 = sizeof(T) == 8 && g_update_fp32.load(std::memory_order_relaxed);
+            switch (g_gemm_stage.load(std::memory_order_relaxed)) {
+            case 16:
+                if (f32) hipLaunchKernelGGL((k_gemm_b<T, 16, float2>), grid, dim3(256), 0, s, d, j0);
+                else hipLaunchKernelGGL((k_gemm_b<T, 16>), grid, dim3(256), 0, s, d, j0);
+                break;
+            case 32:
+                if (f32) hipLaunchKernelGGL((k_gemm_b<T, 32, float2>), grid, dim3(256), 0, s, d, j0);
+                else hipLaunchKernelGGL((k_gemm_b<T, 32>), grid, dim3(256), 0, s, d, j0);
+                break;
+            default:
+                if (f32) hipLaunchKernelGGL((k_gemm_b<T, 8, float2>), grid, dim3(256), 0, s, d, j0);
+                else hipLaunchKernelGGL((k_gemm_b<T, 8>), grid, dim3(256), 0, s, d, j0);
+            }
         }
         if (int rc = check_launch("getrf_batched/update")) return rc;
     }
@@ -1344,6 +1390,22 @@ int getrs_noalloc_t(const T *LU, long n, long lda, const int *ipiv, const int *p
 }  // namespace
 
 extern "C" {
+
+int m25x_lu_set_gemm_stage(int ks)
+{
+    if (ks != 8 && ks != 16 && ks != 32) return fail(M25X_EINVAL, "lu_set_gemm_stage: 8, 16 or 32, not %d", ks);
+    g_gemm_stage.store(ks, std::memory_order_relaxed);
+    return M25X_OK;
+}
+
+int m25x_lu_gemm_stage(void) { return g_gemm_stage.load(std::memory_order_relaxed); }
+
+int m25x_lu_set_update_precision(int fp32)
+{
+    if (fp32 != 0 && fp32 != 1) return fail(M25X_EINVAL, "lu_set_update_precision: 0 (FP64) or 1 (FP32), not %d", fp32);
+    g_update_fp32.store(fp32, std::memory_order_relaxed);
+    return M25X_OK;
+}
 
 int m25x_getrf_batched(int dtype, int nitems, const m25x_lu_item *items_host, const m25x_lu_item *items_dev,
                        void *stream)

@@ -232,12 +232,14 @@ int check_requests(const m25x_stack *st, const int *ri, int nreq)
 {
     for (int r = 0; r < nreq; ++r) {
         const int *q = ri + M25X_SPEC_NI * r;
-        const int top = st->N;
-        for (int c = 2; c < 6; ++c)
-            if (q[c] < 0 || q[c] > top) return fail(M25X_EINVAL, "spectra: request %d names interface %d", r, q[c]);
+        if (q[0] < M25X_SPEC_TABLE || q[0] > M25X_SPEC_VIA_R) return fail(M25X_EINVAL, "spectra: request type %d", q[0]);
+        // the interfaces each type names: a table o, s; a via kernel k, lo, hi; a via reaction four span ends
+        const int nif = q[0] == M25X_SPEC_TABLE ? 2 : (q[0] == M25X_SPEC_VIA_D ? 3 : 4);
+        for (int c = 2; c < 2 + nif; ++c)
+            if (q[c] < 0 || q[c] > st->N)
+                return fail(M25X_EINVAL, "spectra: request %d names interface %d", r, q[c]);
         if (q[0] == M25X_SPEC_TABLE && (q[4] < 0 || q[4] > 2))
             return fail(M25X_EINVAL, "spectra: request %d extracts %d parts (0 to 2)", r, q[4]);
-        if (q[0] < M25X_SPEC_TABLE || q[0] > M25X_SPEC_VIA_R) return fail(M25X_EINVAL, "spectra: request type %d", q[0]);
         if (q[1] < 0) return fail(M25X_EINVAL, "spectra: request %d: output row %d", r, q[1]);
     }
     return M25X_OK;

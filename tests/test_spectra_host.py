@@ -74,3 +74,20 @@ def test_bad_stacks_are_refused():
     st = LayerStack([Layer(1e-3, 4.0)])
     with pytest.raises(ValueError):
         st.spectral_point(100.0, 2 * np.pi * 1e9, 0, 5)
+
+
+def test_requests_are_checked_by_type():
+    """A one-layer table request that extracts two parts (its count is not an interface) is accepted; a request
+    that names an interface beyond the stack is refused before any launch (no GPU needed: the check runs first)."""
+    import ctypes as C
+    from mom25d_amd import _lib as L
+    from mom25d_amd.multilayer import Layer, LayerStack
+    st = LayerStack([Layer(1e-3, 4.0)])
+    w = 2 * np.pi * 1e9
+    ri = np.array([[L.SPEC_TABLE, 0, 1, 1, 2, 0, 0, 0]], np.int32)
+    bad = np.array([[L.SPEC_VIA_D, 0, 1, 0, 2, 0, 0, 0]], np.int32)
+    z = C.c_void_p(8)                          # never dereferenced: the call stops at the check or at npts = 0
+    L.call('m25x_spectra', C.byref(st.struct(w)), w, z, z, 0, z, z, 1, ri.ctypes.data_as(C.c_void_p), z, 2, None)
+    with pytest.raises(ValueError, match='interface 2'):
+        L.call('m25x_spectra', C.byref(st.struct(w)), w, z, z, 1, z, z, 1, bad.ctypes.data_as(C.c_void_p), z, 2,
+               None)
