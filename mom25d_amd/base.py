@@ -4,7 +4,8 @@
                   (b, c), (c, a); T+ the first triangle seen; alpha = +-l / 2A; the free vertices
   _BAR6/_W6, ...  src/mom25d/base.py:70-77: the 6-point degree-4 and 3-point degree-2 rules
   geometry        src/mom25d/fast.py:375-400 and src/mom25d/sim.py:543-548: the kernels' g / q dicts
-  fit_gamma/waves src/mom25d/base.py:161-172: the travelling-wave fits of a wave port's feed
+  gram_entries    the RWG Gram matrix G_mn = int f_m . f_n dS (src/mom25d/base.py:153-157), exactly, as sparse
+                  entries (the surface-impedance term Zs G of conductor loss)
 """
 import numpy as np
 
@@ -106,15 +107,23 @@ def geometry(rwg, tri_layer, layers):
     return g, q, size
 
 
-def fit_gamma(Ik, step):
-    """Propagation constant from uniformly spaced samples of a 2-exponential current (base.py:161-165)."""
-    c = np.sum((Ik[:-2] + Ik[2:]) * np.conj(Ik[1:-1])) / (2 * np.sum(np.abs(Ik[1:-1]) ** 2))
-    g = np.arccosh(c + 0j) / step
-    return abs(g.real) + 1j * abs(g.imag)
-
-
-def fit_waves(Ik, s, gamma):
-    """I(s) = a e^{-g s} + b e^{+g s} -> (a, b) per excitation (base.py:168-172)."""
-    M = np.stack([np.exp(-gamma * s), np.exp(gamma * s)], 1)
-    coef, *_ = np.linalg.lstsq(M, Ik, rcond=None)
-    return coef[0], coef[1]
+def gram_entries(g, nodes, tris):
+    """(rows, cols, vals) of the nonzero entries of G_mn = int f_m . f_n dS, each (m, n) once: a sum over the
+    triangles the two functions share of alpha_m alpha_n int_T (r - v_m) . (r - v_n) dA, with the integral exact
+    (a quadratic over the triangle: int_T |r|^2 = A / 12 (sum_k |p_k|^2 + |sum_k p_k|^2), int_T r = A c)."""
+    te, ta = np.asarray(g['te']), np.asarray(g['ta'])
+    tv = np.asarray(g['tv']).reshape(-1, 3, 2)
+    A, c = np.asarray(g['area']), np.asarray(g['cent'])
+    P = nodes[tris]
+    r2 = A / 12 * ((P ** 2).sum((1, 2)) + (P.sum(1) ** 2).sum(1))
+    rows, cols, vals = [], [], []
+    for s in range(3):
+        for u in range(3):
+            on = (te[:, s] >= 0) & (te[:, u] >= 0)
+            vs, vu = tv[on, s], tv[on, u]
+            v = ta[on, s] * ta[on, u] * (r2[on] - A[on] * ((vs + vu) * c[on]).sum(1) + A[on] * (vs * vu).sum(1))
+            rows.append(te[on, s]); cols.append(te[on, u]); vals.append(v)
+    rows, cols, vals = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    Ne = len(g['ep'])
+    key, inv = np.unique(rows.astype(np.int64) * Ne + cols, return_inverse=True)
+    return (key // Ne).astype(np.int64), (key % Ne).astype(np.int64), np.bincount(inv, weights=vals)

@@ -180,8 +180,10 @@ class FastOperator:
     def lean(self):
         """The lean mixed path (the reference's streamed solve, fast.py:1120-1224, in this build's form): only the
         complex64 Z is stored (8 N^2 bytes instead of 24 N^2), and the refinement multiplies by the FP64 Z without
-        forming it (matvec_exact). sim.stream 'on' asks for it."""
-        return getattr(self.sim, 'stream', 'off') == 'on'
+        forming it (matvec_exact). sim.stream 'on' asks for it; 'auto' takes it where the memory plan chose it
+        (the hybrid path's FP64 Z does not fit the device)."""
+        mode = getattr(self.sim, 'stream', 'off')
+        return mode == 'on' or (mode == 'auto' and getattr(self.sim, 'plan', {}).get('path') == 'gpu-stream')
 
     def _solve_lean(self, f, E, t_fresh):
         Z64 = self.fill(f, dv.c64)

@@ -1,10 +1,11 @@
 """S-parameters of a Simulation over a band at the output reference impedances (src/mom25d/run.py:14-84).
 
-    f, S, info = solve_band(sim, 1e9, 6e9, linear=101, zref=50)
+    f, S, info = solve_band(sim, 1e9, 6e9, points=401, zref=50)          # adaptive: a rational model, few solves
+    f, S, info = solve_band(sim, 1e9, 6e9, linear=101, zref=50)          # 101 solves
 
-Every frequency point is one GPU solve (tables, fill, near field, vias, factor or kept factor, refinement);
-S is renormalised from each port's own impedance to zref. The reference's default adaptive sweep (a rational fit
-through emsdev.afs, not available offline) is not built: ask for `linear` points.
+Every frequency point solved is one GPU solve (tables, fill, near field, vias, factor or kept factor, refinement);
+S is renormalised from each port's own impedance to zref. The adaptive sweep is mom25d_amd.afs (the build's own
+AAA sweep in place of emsdev.afs).
 """
 import time
 
@@ -27,9 +28,15 @@ def renorm(S, zc, zref):
 
 
 def out_refs(sim, z0):
-    """Output reference per port: z0 (a number, or {port name: ohms}) (run.py:26-33; lumped ports have no modes)."""
+    """Output reference per port: z0 (a number, {port name: ohms} or one value a port) (run.py:26-33; lumped ports
+    have no modes)."""
     if isinstance(z0, dict):
         return np.array([float(z0.get(p.name, 50.0)) for p in sim.ports])
+    if np.ndim(z0):
+        z = np.asarray(z0, float)
+        if z.shape != (len(sim.ports),):
+            raise ValueError(f"zref: {len(z)} values for {len(sim.ports)} ports")
+        return z
     return np.full(len(sim.ports), float(z0))
 
 
@@ -37,26 +44,41 @@ def _zc(sim, f):
     return np.array([p.zc_fn(f) if getattr(p, 'zc_fn', None) else p.zc for p in sim.ports], float)
 
 
-def solve_band(sim, fmin, fmax, points=401, zref=50.0, linear=None, log=print, progress=None, keep_at=(),
-               freq=None):
-    """(f, S (nf, n, n), info) of sim over fmin..fmax (Hz) (run.py:41-84): `linear` equally spaced points, or the
-    frequencies `freq` when given. info: solves, seconds, zref, the solved frequencies, port names. keep_at:
-    frequencies whose currents are kept in sim.kept. progress(solves, total, f) after each solve."""
-    if freq is None and not linear:
-        raise ValueError("solve_band: the adaptive sweep is not available in this build; pass linear=N (or freq=)")
+def solve_band(sim, fmin, fmax, points=401, zref=50.0, linear=None, afs_tol=2e-3, max_solves=40, log=print,
+               keep_mid=False, progress=None, on_step=None, keep_at=(), freq=None):
+    """(f, S (nf, n, n), info) of sim over fmin..fmax (Hz) (run.py:41-84). Adaptive by default: a rational model
+    through as few solves as afs_tol (max |dS| between successive models, and of the model's prediction of each new
+    sample) allows, at most max_solves, evaluated on `points` equally spaced frequencies, or on `freq` when given
+    (a geometric `freq` is modelled in log f); linear=N solves N equally spaced frequencies instead. keep_mid: one
+    more solve at mid-band keeping its currents (sim.last). keep_at: frequencies the sweep solves among its samples
+    and keeps the currents of (sim.kept). progress(solves, at most, f) after each solve; on_step(entry) after each
+    model. info: solves, seconds, zref, the solved frequencies, port names, the adaptive sweep's history and tol."""
     t0, zr, done = time.time(), out_refs(sim, zref), []
     keep_at = {float(f) for f in keep_at}
     sim.kept = {}
-    fs = np.asarray(freq, float) if freq is not None else np.linspace(fmin, fmax, int(linear))
-    out = []
-    for fr in fs:
+
+    def solve(fr):
         log(f"  solving {fr / 1e9:.4f} GHz")
         keep = float(fr) in keep_at
-        out.append(renorm(sim.sparams(fr, keep=keep), _zc(sim, fr), zr))
+        S = renorm(sim.sparams(fr, keep=keep), _zc(sim, fr), zr)
         if keep:
             sim.kept[float(fr)] = sim.last['I']
         done.append(float(fr))
         if progress:
-            progress(len(done), len(fs), fr)
-    return fs, np.array(out), dict(solves=len(fs), seconds=time.time() - t0, zref=zr.tolist(), solved=done,
-                                   names=[p.name for p in sim.ports], afs=[], afs_tol=None)
+            progress(len(done), int(linear) if linear else max_solves, fr)
+        return S
+    hist = []
+    if linear:
+        f = fs = np.linspace(fmin, fmax, int(linear))
+        S = np.array([solve(fi) for fi in f])
+    else:
+        from .afs import adaptive_sweep
+        f, S, fs, _, hist = adaptive_sweep(solve, fmin, fmax, tol=afs_tol, ngrid=points, max_points=max_solves,
+                                           log=log, on_step=on_step, seed=sorted(keep_at), grid=freq)
+        if hist and not hist[-1]['converged']:
+            sim.log.note(f"WARNING adaptive sweep: stopped at its {max_solves} solves before its models agreed (last "
+                         f"change {hist[-1]['change']:.1e}, tolerance {afs_tol:g}): more solves or a narrower band")
+    if keep_mid:
+        sim.sparams(fs[len(fs) // 2], keep=True)
+    return f, S, dict(solves=len(fs), seconds=time.time() - t0, zref=zr.tolist(), solved=list(map(float, fs)),
+                      names=[p.name for p in sim.ports], afs=hist, afs_tol=None if linear else afs_tol)

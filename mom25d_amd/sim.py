@@ -1,21 +1,21 @@
-"""The simulation object (src/mom25d/sim.py): a stack, Manhattan copper on its interfaces, vias, lumped ports and
-elements -> a mesh, the RWG and via-chain unknowns, and S-parameters per frequency.
+"""The simulation object: a layered stack, Manhattan copper on its interfaces, vias, lumped ports and elements ->
+a mesh, the RWG and via-chain unknowns, and S-parameters per frequency (the reference's src/mom25d/sim.py, for
+boards given as rectangles).
 
     sim = Simulation(stack, {1: [(x0, y0, x1, y1), ...]}, ports=[ViaPort(Via(x, y, 0, 1))], dmax=0.3e-3)
     S = sim.sparams(f)
 
-Kept from the reference: the constructor's arguments that apply to rectangle boards, the unknowns (edges, then
-one per via chain), the fans and their divergence dF, the Green tables per interface pair (green_tables), the via
-blocks (_via_blocks), the excitation and port currents, sparams, sweep, balance and memory_plan. The solve of each
-frequency is mom25d_amd.fast.FastOperator on the GPU. Out of scope here: polygon meshing (shapely/CDT), wave ports
-(LinePort / DiffPort with their wave fits), metal loss, PEC planes with holes or slots, quads and line functions.
+The unknowns are the mesh's RWG edges, then one current per via chain (a via split where it touches metal). Every
+frequency is solved on the GPU by mom25d_amd.fast.FastOperator. The constructor takes the reference's arguments;
+those that ask for what a rectangle board does not have (polygon meshers, quad cells, line functions, zones, PEC
+planes with holes or slots) or for a CPU solver raise ValueError.
 """
 from types import SimpleNamespace
 
 import numpy as np
 import torch
 
-from .base import MU0, RWG, geometry
+from .base import MU0, RWG, geometry, gram_entries
 from .mesh import contains, find_node, tensor_mesh
 from .multilayer import RunLog, device_tables, green_tables
 from . import device as dv
@@ -25,26 +25,24 @@ REUSE_KEEP = 4            # kept factors a solver holds at most (mom25d_amd.fast
 WORK_BYTES = 0.4e9        # the LU's and the solves' workspaces, the tables, the mesh and the near plan, bounded
 
 
-def _zfun(Z):
+def _as_function(Z):
     return Z if callable(Z) else (lambda f, Z=Z: complex(Z))
 
 
 def rlc(R=0.0, L=0.0, C=None, parallel=False):
-    """The impedance function f -> Z of an R-L-C element, series (default) or parallel (the reference's rlc,
-    src/mom25d/sim.py:47-54)."""
+    """f -> Z of a resistor, inductor and capacitor in series (default) or in parallel (sim.py:47-54)."""
     def series(f):
-        jw = 2j * np.pi * f
-        return R + jw * L + (1 / (jw * C) if C else 0)
+        s = 2j * np.pi * f
+        return R + s * L + (1 / (s * C) if C else 0)
 
     def shunt(f):
-        jw = 2j * np.pi * f
-        y = (1 / R if R else 0) + (1 / (jw * L) if L else 0) + (jw * C if C else 0)
-        return 1 / y
+        s = 2j * np.pi * f
+        return 1 / ((1 / R if R else 0) + (1 / (s * L) if L else 0) + (s * C if C else 0))
     return shunt if parallel else series
 
 
 def available_memory():
-    """Bytes of host memory available now (MemAvailable), the reference's emsdev.memory.available."""
+    """Bytes of host memory available now (the kernel's MemAvailable)."""
     try:
         with open('/proc/meminfo') as fh:
             for line in fh:
@@ -58,11 +56,13 @@ def available_memory():
 def device_bytes(n, precision, keep=REUSE_KEEP):
     """Device bytes one n-unknown solve holds on the GPU in this build, where every matrix stays there: 'double' -
     the complex128 Z factored in place (16 n^2); 'mixed' - the FP64 Z the refinement multiplies by (16 n^2), the
-    complex64 factor (8 n^2) and up to `keep` kept factors of nearby frequencies (8 n^2 each, complex128 ones 16);
-    plus WORK_BYTES."""
+    complex64 factor (8 n^2) and up to `keep` kept factors of nearby frequencies (8 n^2 each); 'lean' - the complex64
+    Z factored in place and the kept factors (the refinement without Z); plus WORK_BYTES."""
     nn = float(n) * n
     if precision == 'double':
         return 16.0 * nn + WORK_BYTES
+    if precision == 'lean':
+        return 8.0 * nn * (1 + keep) + WORK_BYTES
     return 24.0 * nn + 8.0 * nn * keep + WORK_BYTES
 
 
@@ -73,12 +73,12 @@ def memory_plan(n, has_core=True, avail=None, gpu=None, precision='auto', solver
 
     gpu: None, (name, free bytes) or (name, free bytes, the LU's bytes, fits) as the reference's gpu_info gives.
     precision 'auto': 'mixed' above GPU_MIN_N unknowns of the LU (n_lu, default n), else 'double'; a requested
-    precision is kept. path: 'hybrid' (complex64 LU and FP64 refinement, both on the GPU) or 'double' (complex128
-    LU on the GPU). gpu (the result): the device bytes of device_bytes(); host: the bytes the host holds (the
-    mesh, geometry and currents, O(n)); fits: whether the device bytes fit 0.9 x the free device memory. The
-    solver is 'direct' (dense LU) whatever is asked: the compressed solvers (BLR, ACA) and the host paths
-    (cpu-mixed, gpu-stream) are not part of this build, and a request that does not fit is reported as such
-    (fits False) for the caller to queue or refuse."""
+    precision is kept. path: 'hybrid' (complex64 LU and FP64 refinement with the FP64 Z on the GPU), 'gpu-stream'
+    (mixed, only the complex64 Z stored: the refinement multiplies without Z - where the hybrid path does not fit
+    and this one does) or 'double' (complex128 LU). gpu (the result): the device bytes; host: the bytes the host
+    holds (the mesh, geometry and currents, O(n)); fits: whether the device bytes fit 0.9 x the free device memory.
+    The solver is 'direct' (dense LU) whatever is asked: the compressed solvers (BLR, ACA) are not part of this
+    build, and a request that does not fit is reported as such (fits False) for the caller to queue or refuse."""
     if n < 0:
         raise ValueError("memory_plan: n must be >= 0")
     free = float(gpu[1]) if gpu else 0.0
@@ -90,50 +90,53 @@ def memory_plan(n, has_core=True, avail=None, gpu=None, precision='auto', solver
     dev = device_bytes(n, precision, keep)
     if gpu and len(gpu) >= 4 and precision == 'mixed':
         dev = max(dev, float(gpu[2]) + 16.0 * n * n)        # the backend's own estimate of its factor, and Z
-    fits = gpu is not None and dev <= 0.9 * free
     path = 'hybrid' if precision == 'mixed' else 'double'
+    if precision == 'mixed' and gpu is not None and dev > 0.9 * free:
+        lean = device_bytes(n, 'lean', keep)
+        if lean <= 0.9 * free:
+            path, dev = 'gpu-stream', lean
+    fits = gpu is not None and dev <= 0.9 * free
     host = 2e4 * n + 64 * 16.0 * n + 0.2e9
     return dict(solver='direct', precision=precision, path=path, host=host, gpu=dev, fits=fits)
 
 
+# ---- board objects ------------------------------------------------------------------------------------------
 class Via:
-    """Plated via on axis (x, y) from interface k_bot to k_top, barrel radius; Z: an element in its lowest chain
-    (sim.py:154-160)."""
+    """A plated via on axis (x, y) from interface k_bot up to k_top, barrel radius; Z: an element in its lowest
+    chain (sim.py:154-160)."""
 
     def __init__(self, x, y, k_bot, k_top, radius=0.15e-3, Z=None, name='', ring=True):
         self.x, self.y, self.k_bot, self.k_top, self.a, self.name = x, y, k_bot, k_top, radius, name
         self.ring = ring
-        self.Z = None if Z is None else _zfun(Z)
+        self.Z = None if Z is None else _as_function(Z)
 
 
 class LinePort:
-    """The transverse line of a gap or element across a feed (geo.py:24-60): layer, center, direction into the
-    device, width; s along the direction."""
+    """A straight line across a feed on one interface: centre, unit direction d into the device, t = d turned by 90
+    degrees, width (geo.py:24-60)."""
     kind = 'line'
 
     def __init__(self, layer, center, direction, width, ref_dist=0.0, name=''):
         d = np.asarray(direction, float)
-        self.layer, self.c, self.d = layer, np.asarray(center, float), d / np.linalg.norm(d)
+        self.d = d / np.linalg.norm(d)
         self.t = np.array([-self.d[1], self.d[0]])
-        self.w, self.ref, self.name = width, ref_dist, name
+        self.layer, self.c, self.w, self.ref, self.name = layer, np.asarray(center, float), width, ref_dist, name
 
     @property
     def axis_aligned(self):
-        return abs(self.d[0]) < 1e-12 or abs(self.d[1]) < 1e-12
+        return bool(np.isclose(self.d, 0.0, atol=1e-12).any())
 
 
 class GapPort:
-    """Lumped port across a strip (delta gap); reference plane at the gap (sim.py:163-175)."""
+    """A lumped port across a strip (a delta gap at its line) (sim.py:163-175)."""
     kind = 'gap'
 
     def __init__(self, layer, center, direction, width, z0=50.0, name=''):
-        self.lp = LinePort(layer, center, direction, width, 0.0)
-        self.layer, self.zc, self.name = layer, z0, name
+        self.lp, self.layer, self.zc, self.name = LinePort(layer, center, direction, width), layer, z0, name
 
 
 class ViaPort:
-    """Lumped port inside a via: an EMF in its lowest chain, + toward the top (sign=-1: toward the bottom)
-    (sim.py:178-184)."""
+    """A lumped port in a via: an EMF in its lowest chain, driving current up (sign -1: down) (sim.py:178-184)."""
     kind = 'via'
 
     def __init__(self, via, z0=50.0, name='', sign=1):
@@ -141,155 +144,224 @@ class ViaPort:
 
 
 class SeriesLumped:
-    """Series element across a strip (sim.py:221-225)."""
+    """An element in series across a strip, at its line (sim.py:221-225)."""
 
     def __init__(self, layer, center, direction, width, Z, name=''):
-        self.lp = LinePort(layer, center, direction, width, 0.0)
-        self.layer, self.Z, self.name = layer, _zfun(Z), name
+        self.lp, self.layer, self.Z, self.name = LinePort(layer, center, direction, width), layer, _as_function(Z), name
+
+
+class Metal:
+    """Conductor loss through a surface impedance (sim.py:228-270): a sheet of conductivity sigma and thickness t
+    has Zs = (k / sigma) coth(k t), k = (1 + j) / delta (1 / (sigma t) at DC, (1 + j) / (sigma delta) when thick).
+    rough (m RMS): the resistance scaled by Hammerstad and Jensen's 1 + (2 / pi) atan(1.4 (rough / delta)^2).
+    finish [(sigma, t), ...] outermost first, on the interfaces of `on` ({interface: share}): there Zs is scaled by
+    1 + share (F - 1), F the finished face's impedance over the bare face's (each finish layer a line section of
+    impedance k_i / sigma_i loading the one beneath it)."""
+
+    def __init__(self, sigma=5.8e7, t=35e-6, rough=0.0, finish=(), on=None):
+        self.sigma, self.t, self.rough = sigma, t, rough
+        self.finish, self.on = [tuple(x) for x in finish], dict(on or {})
+
+    @staticmethod
+    def _k(f, sigma):
+        return (1 + 1j) * np.sqrt(np.pi * f * MU0 * sigma)
+
+    def _bare(self, f):
+        k = self._k(f, self.sigma)
+        return k / self.sigma / np.tanh(k * self.t)
+
+    def zs(self, f, k=None):
+        """Zs (ohms) at f; k: the interface (its finish, if any)."""
+        z = self._bare(f)
+        if self.rough:
+            depth = 1 / np.sqrt(np.pi * f * MU0 * self.sigma)
+            z = z.real * (1 + 2 / np.pi * np.arctan(1.4 * (self.rough / depth) ** 2)) + 1j * z.imag
+        share = self.on.get(k, 0.0) if self.finish else 0.0
+        return z * (1 + share * (self.face_factor(f) - 1)) if share else z
+
+    def face_factor(self, f):
+        """The finished face's impedance over the bare copper face's at f."""
+        bare = z = self._bare(f)
+        for sigma, t in reversed(self.finish):           # inner finish layers first
+            zc, th = self._k(f, sigma) / sigma, np.tanh(self._k(f, sigma) * t)
+            z = zc * (z + zc * th) / (zc + z * th)
+        return z / bare
+
+    def zs_edges(self, f, edge_layer):
+        """Zs of every RWG edge, by its interface: one number unless a finish covers some interfaces."""
+        if not (self.finish and self.on):
+            return self.zs(f)
+        return np.array([self.zs(f, k) if k in self.on else self.zs(f) for k in edge_layer], complex)
 
 
 BALANCE_FLOOR = 1e-2
 
 
 def balance(S, exact=None):
-    """S made reciprocal and the ports' scales (sim.py:1121-1141)."""
-    A = np.abs(S)
-    W = A * A.T
-    np.fill_diagonal(W, 0.0)
+    """(S made reciprocal, port scales d): S_ij d_j / d_i with the log-scales d chosen by weighted least squares so
+    that the scaled S_ij and S_ji agree in magnitude (weights |S_ij S_ji|, a floor that keeps d near 1 where a port
+    couples weakly; `exact` ports keep d = 1), then the symmetric part (sim.py:1121-1141)."""
+    mag = np.abs(S)
+    wgt = mag * mag.T
+    np.fill_diagonal(wgt, 0.0)
     with np.errstate(divide='ignore', invalid='ignore'):
-        R = np.where(W > 0, 0.5 * np.log(A / A.T), 0.0)
-    L = np.diag(W.sum(1) + BALANCE_FLOOR) - W
+        half_log = np.where(wgt > 0, 0.5 * np.log(mag / mag.T), 0.0)
+    lap = np.diag(wgt.sum(1) + BALANCE_FLOOR) - wgt
     if exact is not None and np.any(exact):
-        L = L + np.diag(np.where(exact, 1e6, 0.0))
-    d = np.exp(np.linalg.solve(L, (W * R).sum(1)))
-    S = S * (d[None, :] / d[:, None])
-    return 0.5 * (S + S.T), d
+        lap = lap + np.diag(np.where(exact, 1e6, 0.0))
+    d = np.exp(np.linalg.solve(lap, (wgt * half_log).sum(1)))
+    Sb = S * d[None, :] / d[:, None]
+    return 0.5 * (Sb + Sb.T), d
+
+
+# ---- the simulation -----------------------------------------------------------------------------------------
+_UNSUPPORTED = dict(quads=False, zone=None, dmax_out=None, line_basis=None, coarsen=None, frills=(), slots=(),
+                    dmax_slot=None)
 
 
 class Simulation:
     """A board of rectangles on the stack's interfaces, meshed and set up for the direct solver on the GPU."""
 
     def __init__(self, stack, shapes, ports=(), vias=(), lumped=(), dmax=0.3e-3, dmax_layer=None, w_ref=None,
-                 ncross=6, grade=1.0, near_fac=5.0, solver='direct', precision='auto', gpu='auto', verbose=True,
-                 title='', mesh_only=False, metal=None):
-        if metal is not None:
-            raise ValueError("metal loss is not supported by this build (lossless conductors)")
-        if solver not in ('direct', 'auto'):
-            raise ValueError(f"solver {solver!r}: this build has the direct (dense LU) solver only")
-        self.st, self.title, self.metal, self.log = stack, title, None, RunLog(verbose)
-        self.gpu, self.reuse, self.stream = gpu, 'auto', 'off'
+                 ncross=6, grade=1.0, near_fac=5.0, mesher='auto', metal=None, solver='dense', aca_tol=1e-4,
+                 refine=1.0e-3, precision='auto', gpu='auto', verbose=True, title='', mesh_only=False, core='auto',
+                 quads=False, quad_tol=35.0, zone=None, dmax_out=None, signal_vias='round', stitching_vias='auto',
+                 signal=None, via_barrel=True, line_basis=None, line_w_max=2.0e-3, line_step=2.0, coarsen=None,
+                 grow=0.5, structured='auto', gap_rings=(1 / 4, 1 / 8, 1 / 16), frills=(), slots=(), dmax_slot=None):
+        asked = dict(quads=quads, zone=zone, dmax_out=dmax_out, line_basis=line_basis, coarsen=coarsen,
+                     frills=tuple(frills), slots=tuple(slots), dmax_slot=dmax_slot)
+        for key, value in asked.items():
+            if value != _UNSUPPORTED[key] and not (isinstance(value, tuple) and not value):
+                raise ValueError(f"Simulation: {key}={value!r} is not supported by this build (rectangle boards, "
+                                 "RWG edges and via chains only)")
+        if mesher not in ('auto', 'tensor'):
+            raise ValueError(f"Simulation: mesher {mesher!r}: this build meshes rectangles with a tensor grid")
+        if solver not in ('dense', 'direct', 'auto'):
+            raise ValueError(f"Simulation: solver {solver!r}: this build has the direct (dense LU) solver only")
+        if gpu in ('off', False):
+            raise ValueError("Simulation: gpu='off' - this build solves on the GPU only")
+        if precision not in ('auto', 'mixed', 'double'):
+            raise ValueError(f"Simulation: precision {precision!r}")
+        self.st, self.title, self.metal, self.log = stack, title, metal, RunLog(verbose)
+        self.gpu, self.reuse, self.stream = gpu, 'auto', 'auto'
         self.dmax, self.ncross, self.grade, self.near_fac = dmax, ncross, grade, near_fac
-        self.ports = list(ports)
-        self.vias, self.lumped = list(vias), list(lumped)
-        self.vias += [p.via for p in self.ports if isinstance(p, ViaPort) and p.via not in self.vias]
-        self.layers = sorted(shapes)
-        for k in self.layers:
-            if k not in stack.metal_interfaces:
-                raise ValueError(f"interface {k} cannot carry metal in this stack")
-        self.shapes = {k: [tuple(map(float, r)) for r in shapes[k]] for k in self.layers}
-        dmax_layer = dmax_layer or {}
-        # ---- via chains: split where the via touches metal (sim.py:421-434) ----------------------------------
-        self.chains, self.via_attach = [], []
-        for v in self.vias:
-            on = lambda k: k in self.shapes and contains(self.shapes[k], v.x, v.y)
-            for k in (v.k_bot, v.k_top):
-                if not (stack.is_pec(k) or on(k)):
-                    raise ValueError(f"via {v.name}: ends on M{k} where there is no metal")
-            att = [v.k_bot] + [k for k in range(v.k_bot + 1, v.k_top) if on(k)] + [v.k_top]
-            self.via_attach.append([k for k in att if k in self.shapes])
-            self.chains += [(v, a, b) for a, b in zip(att[:-1], att[1:])]
-        # ---- lines the mesh must follow: gap ports, elements, via axes --------------------------------------
-        lines = {k: [] for k in self.layers}
+        self.ports, self.lumped, self.vias = list(ports), list(lumped), list(vias)
         for p in self.ports:
-            if isinstance(p, GapPort):
-                lines[p.layer].append(p.lp)
-        for el in self.lumped:
-            lines[el.layer].append(el.lp)
+            if isinstance(p, ViaPort) and p.via not in self.vias:
+                self.vias.append(p.via)
+        self.layers = sorted(shapes)
+        bad = [k for k in self.layers if k not in stack.metal_interfaces]
+        if bad:
+            raise ValueError(f"interface {bad[0]} cannot carry metal in this stack")
+        self.shapes = {k: [tuple(map(float, r)) for r in shapes[k]] for k in self.layers}
+        self._split_vias()
         with self.log.timer('mesh'):
-            nodes, tris, tl, off, via_vertex = [], [], [], 0, {}
-            for k in self.layers:
-                xl, yl = [], []
-                for lp in lines[k]:
-                    if not lp.axis_aligned:
-                        raise ValueError("gap ports and elements must be axis-aligned on a rectangle board")
-                    (xl if abs(lp.d[0]) > 0.5 else yl).append(lp.c[0] if abs(lp.d[0]) > 0.5 else lp.c[1])
-                vhere = [(i, v) for i, v in enumerate(self.vias) if k in self.via_attach[i]]
-                dm = dmax_layer.get(k, dmax)
-                wr = w_ref.get(k) if isinstance(w_ref, dict) else w_ref
-                nd, tr = tensor_mesh(self.shapes[k], dm, ncross, 1.01 * max(wr or 0, 1.5 * dm), xl, yl, grade=grade,
-                                     points=[(v.x, v.y) for _, v in vhere])
-                for i, v in vhere:
-                    vid = find_node(nd, v.x, v.y)
-                    if vid < 0:
-                        raise ValueError(f"via {v.name}: not on the M{k} mesh")
-                    via_vertex[(i, k)] = vid + off
-                nodes.append(nd)
-                tris.append(tr + off)
-                tl.append(np.full(len(tr), k))
-                off += len(nd)
-            self.rwg = RWG(np.vstack(nodes), np.vstack(tris))
-            self.tri_layer = np.concatenate(tl)
-            self.edge_layer = self.tri_layer[self.rwg.tp]
-        r = self.rwg
-        self.Ne, self.Nv = r.Ne, len(self.chains)
+            self._mesh(dmax_layer or {}, w_ref)
+        self.Ne, self.Nv = self.rwg.Ne, len(self.chains)
         if mesh_only:
             return
-        self.tri_idx = {k: np.nonzero(self.tri_layer == k)[0] for k in self.layers}
-        # ---- fans and their divergence (sim.py:525-535) ------------------------------------------------------
-        self.fan = {key: np.nonzero((r.tris == vid).any(1))[0] for key, vid in via_vertex.items()}
-        self.F = np.array(sorted({t for v in self.fan.values() for t in v}), int)
-        fpos = {t: j for j, t in enumerate(self.F)}
-        self.dF = np.zeros((len(self.F), self.Nv))
-        for n, (v, kb, kt) in enumerate(self.chains):
-            i = self.vias.index(v)
-            for k, div in ((kb, +1.0), (kt, -1.0)):
-                if (i, k) in self.fan:
-                    ts = self.fan[(i, k)]
-                    self.dF[[fpos[t] for t in ts], n] += div / r.area[ts].sum()
-        self.log.note(f"mesh: {len(r.tris)} triangles, {r.Ne} edge + {self.Nv} via unknowns")
-        self.g, self.q, self.tri_size = geometry(r, self.tri_layer, self.layers)
+        self._fans()
+        self.log.note(f"mesh: {len(self.rwg.tris)} triangles, {self.Ne} edge + {self.Nv} via unknowns")
+        self.g, self.q, self.tri_size = geometry(self.rwg, self.tri_layer, self.layers)
         self.P3, self.W3, self.P6, self.W6 = self.q['P3'], self.q['W3'], self.q['P6'], self.q['W6']
-        self.rho_max = 1.05 * np.hypot(*(r.nodes.max(0) - r.nodes.min(0))) + 1e-4
-        self._edge_sets()
+        span = self.rwg.nodes.max(0) - self.rwg.nodes.min(0)
+        self.rho_max = 1.05 * np.hypot(*span) + 1e-4
+        self.gap = [self._edges_on(p.lp, f"port {p.name}") if isinstance(p, GapPort) else None for p in self.ports]
+        self.lumped_edges = [self._edges_on(el.lp, f"lumped element {el.name}") for el in self.lumped]
+        self._gram = None
         n = self.Ne + self.Nv
-        plan = memory_plan(n, gpu=('hip', self._gpu_free()), precision=precision)
-        self.plan, self.precision, self.solver = plan, plan['precision'], 'direct'
-        self.log.note(f"solver: direct on the GPU (N = {n}, {self.precision} precision)")
+        self.plan = memory_plan(n, gpu=('hip', self._gpu_free()), precision=precision)
+        self.precision, self.solver = self.plan['precision'], 'direct'
+        self.log.note(f"solver: direct on the GPU (N = {n}, {self.precision} precision, {self.plan['path']})")
         from .fast import FastOperator
         self.fast = FastOperator(self)
+
+    # ---- set-up -----------------------------------------------------------------------------------------------
+    def _split_vias(self):
+        """Each via into chains between the interfaces where it touches metal (its ends must: on metal or on a PEC
+        plane of the stack); via_attach: per via, the meshed interfaces it attaches to (sim.py:421-434)."""
+        self.chains, self.via_attach = [], []
+        for v in self.vias:
+            def on_metal(k):
+                return k in self.shapes and contains(self.shapes[k], v.x, v.y)
+            for k in (v.k_bot, v.k_top):
+                if not (self.st.is_pec(k) or on_metal(k)):
+                    raise ValueError(f"via {v.name}: ends on M{k} where there is no metal")
+            stops = [v.k_bot] + [k for k in range(v.k_bot + 1, v.k_top) if on_metal(k)] + [v.k_top]
+            self.via_attach.append([k for k in stops if k in self.shapes])
+            self.chains += [(v, lo, hi) for lo, hi in zip(stops[:-1], stops[1:])]
+
+    def _mesh(self, dmax_layer, w_ref):
+        """Each interface's copper meshed with the lines of its gap ports and elements and its via axes as grid
+        lines; the triangles of all interfaces in one RWG basis; the via vertex on each interface a via attaches to."""
+        nodes, tris, layer_of, self._via_vertex = [], [], [], {}
+        base = 0
+        for k in self.layers:
+            across = {0: [], 1: []}                 # ports and elements: a line across x (a constant x) or y
+            feeds = [p.lp for p in self.ports if isinstance(p, GapPort) and p.layer == k] + \
+                [el.lp for el in self.lumped if el.layer == k]
+            for lp in feeds:
+                if not lp.axis_aligned:
+                    raise ValueError("gap ports and elements must be axis-aligned on a rectangle board")
+                axis = 0 if abs(lp.d[0]) > 0.5 else 1
+                across[axis].append(lp.c[axis])
+            here = [(i, v) for i, v in enumerate(self.vias) if k in self.via_attach[i]]
+            dm = dmax_layer.get(k, self.dmax)
+            ref = w_ref.get(k) if isinstance(w_ref, dict) else w_ref
+            nd, tr = tensor_mesh(self.shapes[k], dm, self.ncross, 1.01 * max(ref or 0, 1.5 * dm), across[0],
+                                 across[1], grade=self.grade, points=[(v.x, v.y) for _, v in here])
+            for i, v in here:
+                node = find_node(nd, v.x, v.y)
+                if node < 0:
+                    raise ValueError(f"via {v.name}: not on the M{k} mesh")
+                self._via_vertex[(i, k)] = node + base
+            nodes.append(nd)
+            tris.append(tr + base)
+            layer_of.append(np.full(len(tr), k))
+            base += len(nd)
+        self.rwg = RWG(np.vstack(nodes), np.vstack(tris))
+        self.tri_layer = np.concatenate(layer_of)
+        self.edge_layer = self.tri_layer[self.rwg.tp]
+        self.tri_idx = {k: np.flatnonzero(self.tri_layer == k) for k in self.layers}
+
+    def _fans(self):
+        """The fan of a via on an interface: the triangles around its vertex there. F: every fan triangle once; dF
+        (|F| x Nv): a chain's end charge spread uniformly over its fans, + at its bottom, - at its top (sim.py:
+        525-535)."""
+        r = self.rwg
+        self.fan = {key: np.flatnonzero((r.tris == vtx).any(1)) for key, vtx in self._via_vertex.items()}
+        self.F = np.unique(np.concatenate(list(self.fan.values()))) if self.fan else np.zeros(0, int)
+        row = {t: j for j, t in enumerate(self.F)}
+        self.dF = np.zeros((len(self.F), len(self.chains)))
+        via_index = {id(v): i for i, v in enumerate(self.vias)}
+        for n, (v, lo, hi) in enumerate(self.chains):
+            for k, sign in ((lo, 1.0), (hi, -1.0)):
+                tri = self.fan.get((via_index[id(v)], k))
+                if tri is not None:
+                    self.dF[[row[t] for t in tri], n] += sign / r.area[tri].sum()
+
+    def _edges_on(self, lp, what, tol=1e-8):
+        """(edges, signs) of the RWG edges lying on a feed line within its width, on its interface: the sign is
+        that of the edge's current direction along the feed (sim.py:697-728)."""
+        r = self.rwg
+        rel = r.ends - lp.c[None, None, :]                     # (Ne, 2 ends, 2)
+        along, across = rel @ lp.d, rel @ lp.t
+        on = (np.abs(along) < tol).all(1) & (np.abs(across) <= lp.w / 2 + tol).all(1) & (self.edge_layer == lp.layer)
+        idx = np.flatnonzero(on)
+        if not len(idx):
+            raise ValueError(f"{what}: no mesh edges on its line")
+        return idx, np.sign(r.nrm[idx] @ lp.d)
 
     @staticmethod
     def _gpu_free():
         if not torch.cuda.is_available():
             return 0.0
-        free, _ = torch.cuda.mem_get_info()
-        return float(free)
-
-    # ---- edges on lines (sim.py:697-728) -----------------------------------------------------------------
-    def _line_edges(self, lp, s=0.0, tol=1e-8):
-        r = self.rwg
-        e0, e1 = r.ends[:, 0] - (lp.c + lp.d * s), r.ends[:, 1] - (lp.c + lp.d * s)
-        on = (np.abs(e0 @ lp.d) < tol) & (np.abs(e1 @ lp.d) < tol) & (self.edge_layer == lp.layer)
-        on &= (np.abs(e0 @ lp.t) <= lp.w / 2 + tol) & (np.abs(e1 @ lp.t) <= lp.w / 2 + tol)
-        idx = np.nonzero(on)[0]
-        return idx, np.sign(r.nrm[idx] @ lp.d)
-
-    def _edge_sets(self):
-        self.gap = []
-        for p in self.ports:
-            gap = self._line_edges(p.lp) if isinstance(p, GapPort) else None
-            if gap is not None and not len(gap[0]):
-                raise ValueError(f"port {p.name}: no mesh edges on the gap line")
-            self.gap.append(gap)
-        self.lumped_edges = [self._line_edges(el.lp) for el in self.lumped]
-        for el, e in zip(self.lumped, self.lumped_edges):
-            if not len(e[0]):
-                raise ValueError(f"lumped element {el.name}: no mesh edges on its line")
+        return float(torch.cuda.mem_get_info()[0])
 
     def _port_chain(self, p):
         return next(n for n, (v, _, _) in enumerate(self.chains) if v is p.via)
 
-    # ---- per-frequency parts ------------------------------------------------------------------------------
+    # ---- per-frequency parts -----------------------------------------------------------------------------------
     def green_tables(self, f):
         """{(a, b): GreenTableML} for every pair of metal interfaces at f, on the device (sim.py:736-745)."""
         with self.log.timer('Green tables', f"{len(self.layers)} interfaces @ {f / 1e9:.3f} GHz"):
@@ -300,94 +372,97 @@ class Simulation:
     VIA_TABLE_PAIRS = 64
 
     def _via_geometry(self):
-        """Each chain's axis, the chains of each span, every pair (m <= n) grouped by its two spans with its
-        distance (a chain with its own via: the barrel radius), geometry only (sim.py:786-814)."""
+        """Geometry of the via blocks, once: each chain's axis, the chains of each span, and every chain pair
+        (m <= n) grouped by its two spans with its distance (the barrel radius for two chains of one via, at least
+        the mean radius otherwise) (sim.py:786-814)."""
         if getattr(self, '_via_geo', None) is not None:
             return self._via_geo
         ch = self.chains
-        x = np.array([v.x for v, _, _ in ch], float)
-        y = np.array([v.y for v, _, _ in ch], float)
-        a = np.array([v.a for v, _, _ in ch], float)
-        via = {}
-        vid = np.array([via.setdefault(id(v), len(via)) for v, _, _ in ch], int)
-        span_of = [(kb, kt) for _, kb, kt in ch]
-        spans = sorted(set(span_of))
-        si = np.array([spans.index(s) for s in span_of], int)
-        by_span = {s: np.flatnonzero(si == i) for i, s in enumerate(spans)}
+        x, y, a = (np.array([getattr(v, k) for v, _, _ in ch], float) for k in ('x', 'y', 'a'))
+        owner = np.unique([id(v) for v, _, _ in ch], return_inverse=True)[1].ravel()
+        spans = sorted({(lo, hi) for _, lo, hi in ch})
+        sp = np.array([spans.index((lo, hi)) for _, lo, hi in ch], int)
         m, n = np.triu_indices(len(ch))
-        d = np.where(vid[m] == vid[n], a[m], np.maximum(np.hypot(x[m] - x[n], y[m] - y[n]), 0.5 * (a[m] + a[n])))
-        key = si[m] * len(spans) + si[n]
-        order = np.argsort(key, kind='stable')
+        dist = np.where(owner[m] == owner[n], a[m], np.maximum(np.hypot(x[m] - x[n], y[m] - y[n]), 0.5 * (a[m] + a[n])))
         groups = {}
-        for gk in np.unique(key):
-            sel = order[np.searchsorted(key[order], gk, 'left'):np.searchsorted(key[order], gk, 'right')]
-            groups[(spans[gk // len(spans)], spans[gk % len(spans)])] = (m[sel], n[sel], d[sel])
-        self._via_geo = dict(x=x, y=y, by_span=by_span, groups=groups, spans=spans)
+        for i in range(len(spans)):
+            for j in range(len(spans)):
+                sel = np.flatnonzero((sp[m] == i) & (sp[n] == j))
+                if len(sel):
+                    groups[(spans[i], spans[j])] = (m[sel], n[sel], dist[sel])
+        self._via_geo = dict(x=x, y=y, by_span={s: np.flatnonzero(sp == i) for i, s in enumerate(spans)},
+                             groups=groups, spans=spans)
         return self._via_geo
 
     def _via_blocks(self, f, QF):
         """B (Ne x Nv) and C (Nv x Nv), device complex128 (sim.py:816-872): the fans' end-charge potentials (QF:
         the accurate G_q moments between every triangle and the fan triangles, device Nt x |F|), the smooth
         via -> strip kernels tested with the RWG divergence, the via-via reactions with their K0 part, elements
-        in vias."""
+        in vias and, with metal loss, the barrel's wall loss."""
         from .via import ViaKernels
-        w = 2 * np.pi * f
-        r, Ne, Nv = self.rwg, self.Ne, self.Nv
-        dev = dv.device()
-        B = torch.zeros((Ne, Nv), dtype=dv.c128, device=dev)
-        Ch = np.zeros((Nv, Nv), complex)
-        vk = ViaKernels(f, self.st, [(kb, kt) for _, kb, kt in self.chains], self.layers, self.rho_max)
-        geo = self._via_geometry()
-        ops = self._div_ops()
+        jw = 2j * np.pi * f
+        Nt, dev = len(self.rwg.tris), dv.device()
+        B = torch.zeros((self.Ne, self.Nv), dtype=dv.c128, device=dev)
+        C = np.zeros((self.Nv, self.Nv), complex)
+        vk = ViaKernels(f, self.st, [(lo, hi) for _, lo, hi in self.chains], self.layers, self.rho_max)
+        geo, div = self._via_geometry(), self._div_ops()
         if len(self.F):
-            dF = torch.from_numpy(self.dF).to(dev, dv.c128)
-            T = QF @ dF                                                     # (Nt, Nv) fan-charge potentials
-            B += ops(T) / (1j * w)
-            Ch += (self.dF.T @ T[torch.from_numpy(self.F).to(dev)].cpu().numpy()) / (1j * w)
+            pot = QF @ torch.from_numpy(self.dF).to(dev, dv.c128)            # (Nt, Nv) the end charges' potential
+            B += div(pot) / jw
+            C += (self.dF.T @ pot[torch.from_numpy(self.F).to(dev)].cpu().numpy()) / jw
         for span, ns in geo['by_span'].items():
-            c = torch.zeros((len(r.tris), len(ns)), dtype=dv.c128, device=dev)
+            col = torch.zeros((Nt, len(ns)), dtype=dv.c128, device=dev)
             for k in self.layers:
-                idx = self.tri_idx[k]
-                D = vk.D[(k, span)]
-                c[torch.from_numpy(idx).to(dev)] = dv.radial_table_sum(self.P6[idx, :, 0], self.P6[idx, :, 1],
-                                                                       self.W6[idx], geo['x'][ns], geo['y'][ns],
-                                                                       D.v, D._dr)
-            B[:, torch.from_numpy(ns).to(dev)] += ops(c)
+                idx, D = self.tri_idx[k], vk.D[(k, span)]
+                col[torch.from_numpy(idx).to(dev)] = dv.radial_table_sum(
+                    self.P6[idx, :, 0], self.P6[idx, :, 1], self.W6[idx], geo['x'][ns], geo['y'][ns], D.v, D._dr)
+            B[:, torch.from_numpy(ns).to(dev)] += div(col)
         for (sm, sn), (m, n, dd) in geo['groups'].items():
-            vals = (vk.R_table(sm, sn)(dd) if len(dd) > self.VIA_TABLE_PAIRS else vk.R(sm, sn, dd)) + vk.homog(sm, sn, dd)
-            np.add.at(Ch, (m, n), vals)
+            remainder = vk.R_table(sm, sn)(dd) if len(dd) > self.VIA_TABLE_PAIRS else vk.R(sm, sn, dd)
+            vals = remainder + vk.homog(sm, sn, dd)
+            np.add.at(C, (m, n), vals)
             off = m != n
-            np.add.at(Ch, (n[off], m[off]), vals[off])
-        seen = set()
-        for n, (v, kb, kt) in enumerate(self.chains):
-            if v.Z is not None and id(v) not in seen:
-                Ch[n, n] += v.Z(f)
-            seen.add(id(v))
-        return B, torch.from_numpy(Ch).to(dev)
+            np.add.at(C, (n[off], m[off]), vals[off])
+        zs = self.metal.zs(f) if self.metal is not None else 0.0
+        first = set()
+        for n, (v, lo, hi) in enumerate(self.chains):
+            if v.Z is not None and id(v) not in first:                       # an element: in its lowest chain
+                C[n, n] += v.Z(f)
+            first.add(id(v))
+            C[n, n] += zs * (self.st.z[hi] - self.st.z[lo]) / (2 * np.pi * v.a)
+        return B, torch.from_numpy(C).to(dev)
 
     def _div_ops(self):
-        """x (Nt, k) -> D^T x (Ne, k) on the device: +l/A on T+, -l/A on T- (base.py:47)."""
+        """x (Nt, k) -> D^T x (Ne, k) on the device: each edge's +l / A_+ on T+ and -l / A_- on T- (base.py:47)."""
         if getattr(self, '_div', None) is None:
             r, dev = self.rwg, dv.device()
             tp, tm = torch.from_numpy(r.tp).to(dev), torch.from_numpy(r.tm).to(dev)
-            ap = torch.from_numpy(r.l / r.area[r.tp]).to(dev, dv.c128)[:, None]
-            am = torch.from_numpy(-r.l / r.area[r.tm]).to(dev, dv.c128)[:, None]
-            self._div = lambda x: ap * x[tp] + am * x[tm]
+            wp = torch.from_numpy(r.l / r.area[r.tp]).to(dev, dv.c128)[:, None]
+            wm = torch.from_numpy(-r.l / r.area[r.tm]).to(dev, dv.c128)[:, None]
+            self._div = lambda x: wp * x[tp] + wm * x[tm]
         return self._div
 
     def lumped_entries(self, f):
-        """(rows, cols, values) of the lumped elements' Z_el u u^T (fast.py:628-641)."""
-        rows, cols, vals = [], [], []
+        """(rows, cols, values) added to Z at f: the lumped elements' Z_el u u^T (u: the element's edges, +-l) and,
+        with metal loss, Zs (per edge's interface) times the RWG Gram matrix (fast.py:628-641); None if none."""
+        parts = []
         for el, (idx, sig) in zip(self.lumped, self.lumped_edges):
             u = sig * self.rwg.l[idx]
             R, Cc = np.meshgrid(idx, idx, indexing='ij')
-            rows.append(R.ravel()); cols.append(Cc.ravel()); vals.append((el.Z(f) * np.outer(u, u)).ravel())
-        if not rows:
+            parts.append((R.ravel(), Cc.ravel(), (el.Z(f) * np.outer(u, u)).ravel()))
+        if self.metal is not None:
+            if self._gram is None:
+                self._gram = gram_entries(self.g, self.rwg.nodes, self.rwg.tris)
+            r, c, g = self._gram
+            zs = self.metal.zs_edges(f, self.edge_layer)
+            parts.append((r, c, (zs[r] if np.ndim(zs) else zs) * g))
+        if not parts:
             return None
-        return np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+        return tuple(np.concatenate([p[i] for p in parts]) for i in range(3))
 
-    # ---- excitation and port quantities (sim.py:988-1003) -------------------------------------------------
+    # ---- excitation, currents and S (sim.py:988-1111) ---------------------------------------------------------
     def excitation(self):
+        """One column per port: a via port's EMF in its chain's row (its sign), a gap port's +-l on its edges."""
         E = np.zeros((self.Ne + self.Nv, len(self.ports)), complex)
         for j, p in enumerate(self.ports):
             if isinstance(p, ViaPort):
@@ -398,15 +473,17 @@ class Simulation:
         return E
 
     def port_current(self, I, j):
+        """Port j's current for each excitation column of I: its chain's current (signed) or the current through
+        its gap line."""
         p = self.ports[j]
         if isinstance(p, ViaPort):
             return p.sign * I[self.Ne + self._port_chain(p)]
         idx, sig = self.gap[j]
-        return ((sig * self.rwg.l[idx])[:, None] * I[idx]).sum(0)
+        return (sig * self.rwg.l[idx]) @ I[idx]
 
     def currents(self, f, E):
-        """Basis currents for excitation columns E (sim.py:1014-1031): a result that is not finite raises
-        FloatingPointError (no CPU solver to fall back on)."""
+        """Basis currents for excitation columns E (sim.py:1014-1031); a result that is not finite raises
+        FloatingPointError (there is no CPU solver to fall back on)."""
         I = self.fast.solve_direct(f, E, precision=self.precision)
         if not np.isfinite(I).all():
             raise FloatingPointError(f"mom25d_amd: the direct solve at {f / 1e9:.4g} GHz is not finite "
@@ -418,24 +495,20 @@ class Simulation:
         return self.sparams_from(f, self.currents(f, self.excitation()), keep)
 
     def sparams_from(self, f, I, keep=False):
-        """S at f from the basis currents I of the excitation() columns."""
-        n = len(self.ports)
-        Ap, Bp = np.zeros((n, n), complex), np.zeros((n, n), complex)
-        for i, p in enumerate(self.ports):
-            Vp, Ip = np.eye(n)[i], self.port_current(I, i)
-            Ap[i], Bp[i] = (Vp + p.zc * Ip) / (2 * np.sqrt(p.zc)), (Vp - p.zc * Ip) / (2 * np.sqrt(p.zc))
-        S = Bp @ np.linalg.inv(Ap)
-        self.last = dict(f=f, I=I if keep else None, A=Ap, B=Bp)
+        """S at f from the basis currents I of the excitation() columns: each port driven by 1 V in turn, its
+        incident and reflected waves a = (V + Zc I) / (2 sqrt Zc), b = (V - Zc I) / (2 sqrt Zc), S = B A^-1."""
+        zc = np.array([p.zc for p in self.ports], float)
+        Ip = np.array([self.port_current(I, j) for j in range(len(self.ports))])    # (port, excitation)
+        V = np.eye(len(self.ports))
+        root = (2 * np.sqrt(zc))[:, None]
+        A, B = (V + zc[:, None] * Ip) / root, (V - zc[:, None] * Ip) / root
+        S = B @ np.linalg.inv(A)
+        self.last = dict(f=f, I=I if keep else None, A=A, B=B)
         return S
 
     def sweep(self, freqs):
-        """S, reciprocity and passivity per frequency (sim.py:1094-1111)."""
-        out = SimpleNamespace(f=np.asarray(freqs), S=[], recip=[], passiv=[])
-        for f in freqs:
-            S = self.sparams(f)
-            out.S.append(S)
-            out.recip.append(np.max(np.abs(S - S.T)))
-            out.passiv.append(np.max(np.linalg.svd(S, compute_uv=False)))
-        for k in ('S', 'recip', 'passiv'):
-            setattr(out, k, np.array(getattr(out, k)))
-        return out
+        """S per frequency with its reciprocity (max |S - S^T|) and passivity (largest singular value)
+        (sim.py:1094-1111)."""
+        S = np.array([self.sparams(f) for f in freqs])
+        return SimpleNamespace(f=np.asarray(freqs), S=S, recip=np.abs(S - S.transpose(0, 2, 1)).max((1, 2)),
+                               passiv=np.array([np.linalg.svd(s, compute_uv=False).max() for s in S]))

@@ -105,3 +105,20 @@ def geometry_dicts(rwg, tri_layer, layers):
     V = r.nodes[r.tris]
     size = np.linalg.norm(V - np.roll(V, 1, axis=1), axis=2).max(1)
     return g, q, size
+
+
+def gram_midpoint(rwg):
+    """G_mn = int f_m . f_n dS (dense), each shared triangle by the three-edge-midpoint rule (exact for the
+    quadratic integrand) from the RWG functions' definition (base.py:153-157 integrates the same exactly)."""
+    G = np.zeros((rwg.Ne, rwg.Ne))
+    support = {}
+    for e in range(rwg.Ne):
+        for t, v, s in ((rwg.tp[e], rwg.vp[e], 1.0), (rwg.tm[e], rwg.vm[e], -1.0)):
+            support.setdefault(int(t), []).append((e, v, s * rwg.l[e] / (2 * rwg.area[t])))
+    for t, fs in support.items():
+        P = rwg.nodes[rwg.tris[t]]
+        mids = 0.5 * (P + np.roll(P, -1, axis=0))
+        for m, vm, am in fs:
+            for n, vn, an in fs:
+                G[m, n] += am * an * rwg.area[t] / 3 * ((mids - vm) * (mids - vn)).sum()
+    return G

@@ -32,6 +32,7 @@ class Board:
     E: np.ndarray                   # excitation (N x ports)
     lumped: list = field(default_factory=list)      # [(edge idx, signs, Z(f))]
     near_fac: float = 5.0
+    zs: object = None               # conductor loss: f -> surface impedance (ohms), or None (lossless)
 
 
 class Solver:
@@ -100,10 +101,12 @@ class Solver:
                 off = mm != nn
                 np.add.at(C, (nn[off], mm[off]), vals[off])
         seen = set()
+        zs = b.zs(f) if b.zs is not None else 0.0
         for i, c in enumerate(b.chains):
             if c[6] is not None and c[5] not in seen:
                 C[i, i] += c[6](f)
             seen.add(c[5])
+            C[i, i] += zs * (b.stack.z[c[4]] - b.stack.z[c[3]]) / (2 * np.pi * c[2])      # the barrel's wall loss
         return B, C
 
     def fill(self, f):
@@ -121,6 +124,10 @@ class Solver:
         for idx, sig, zf in b.lumped:
             u = sig * self.rwg.l[idx]
             Z[np.ix_(idx, idx)] += zf(f) * np.outer(u, u)
+        if b.zs is not None:                      # conductor loss: Zs int f_m . f_n dS (fast.py:628-641)
+            if getattr(self, '_gram', None) is None:
+                self._gram = ogeo.gram_midpoint(self.rwg)
+            Z[:self.Ne, :self.Ne] += b.zs(f) * self._gram
         return Z
 
     def solve(self, f, precision='double'):

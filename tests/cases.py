@@ -45,9 +45,10 @@ def oracle_board(sim):
                         bottom=st.bottom, bottom_er=st.bottom_er)
     chains = [(v.x, v.y, v.a, kb, kt, id(v), v.Z) for v, kb, kt in sim.chains]
     lumped = [(idx, sig, el.Z) for el, (idx, sig) in zip(sim.lumped, sim.lumped_edges)]
+    zs = (lambda f, m=sim.metal: complex(m.zs(f))) if sim.metal is not None else None
     return Board(stack=ost, nodes=sim.rwg.nodes, tris=sim.rwg.tris, tri_layer=sim.tri_layer, layers=sim.layers,
                  dmax=sim.dmax, chains=chains, F=sim.F, dF=sim.dF, E=sim.excitation(), lumped=lumped,
-                 near_fac=sim.near_fac)
+                 near_fac=sim.near_fac, zs=zs)
 
 
 def port_rows(sim):

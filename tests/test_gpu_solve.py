@@ -130,3 +130,71 @@ def test_matrix_free_operator_and_lean_path(gpu):
         Sd, Sl = sd.sparams(fr), sl.sparams(fr)
         assert np.abs(Sl - Sd).max() <= 1e-4, (fr, np.abs(Sl - Sd).max())
     assert any('reused' in n for n in sl.log.notes)
+
+
+def test_metal_loss_matches_oracle_and_is_lossy(gpu):
+    """The microstrip with copper loss (5.8e7 S/m, 35 um, 1 um roughness): S against the oracle (its Gram matrix by
+    the midpoint rule, its wall loss its own) to 1e-5; the loss shows: |S21| below the lossless one, the ports'
+    power sum below 1; reciprocal."""
+    from mom25d_amd.sim import Metal
+    metal = Metal(5.8e7, 35e-6, rough=1e-6)
+    st, sh, po, vi, kw = boards.microstrip()
+    lossy = Simulation(st, sh, po, vi, precision='double', verbose=False, metal=metal, **kw)
+    osv = Solver(oracle_board(lossy))
+    rows, signs = port_rows(lossy)
+    zc = [p.zc for p in lossy.ports]
+    plain = make(boards.microstrip, 'double')
+    for f in (2e9, 5e9):
+        So = osparams(osv.solve(f), osv.b.E, zc, rows, signs)
+        Sl, Sp = lossy.sparams(f), plain.sparams(f)
+        assert np.abs(Sl - So).max() <= 1e-5, (f, np.abs(Sl - So).max())
+        assert abs(Sl[1, 0]) < abs(Sp[1, 0]) and np.sum(np.abs(Sl[:, 0]) ** 2) < 1 - 1e-4
+        assert np.abs(Sl - Sl.T).max() < 1e-8
+
+
+def test_solve_board_on_a_rectangle_board(gpu):
+    """solve_board with a rectangle board in the build's JSON form (the microstrip): the adaptive sweep's S equals
+    the simulation's own at the frequencies it solved and stays within its tolerance of a direct solve elsewhere; a
+    kicad2ems-like file is refused."""
+    from mom25d_amd.api import solve_board
+    board = dict(layers=[[0.8e-3, 4.4, 0.02]], shapes={'1': [[-15e-3, -0.75e-3, 15e-3, 0.75e-3]]},
+                 ports=[dict(via=[-15e-3, 0.0, 0, 1, 0.15e-3], name='P1'), dict(via=[15e-3, 0.0, 0, 1, 0.15e-3],
+                                                                              name='P2')],
+                 dmax=0.3e-3, ncross=4)
+    freq = np.linspace(1e9, 6e9, 51)
+    res = solve_board(board, freq, max_solves=16, afs_tol=2e-3, log=lambda *a: None)
+    assert res.S.shape == (51, 2, 2) and res.names == ['P1', 'P2'] and res.solves <= 16
+    sim = res.sim
+    for f in res.solved[:3]:
+        k = int(np.argmin(np.abs(freq - f)))
+        assert np.abs(res.S[k] - sim.sparams(freq[k])).max() <= 1e-9
+    k = 17
+    assert np.abs(res.S[k] - sim.sparams(freq[k])).max() <= 1e-2
+    with pytest.raises(ValueError, match='kicad2ems'):
+        solve_board(dict(nets=[], layers=[]), freq, log=lambda *a: None)
+
+
+def test_series_element_is_a_terminated_port(gpu):
+    """A series resistor across a line (SeriesLumped: Z_el u u^T added to Z) equals the same line with its gap
+    driven as a third port and that port terminated in Z_el by circuit algebra (the Schur complement of the
+    3-port's impedance matrix), to 1e-9: the element's entries, the gap port's excitation and its current are one
+    consistent u on the GPU path as in the oracle's (test_oracle_network)."""
+    from mom25d_amd.multilayer import Layer, LayerStack
+    from mom25d_amd.sim import GapPort, SeriesLumped, Via, ViaPort
+    MM = 1e-3
+    L, w = 10 * MM, 0.35 * MM
+    st = LayerStack([Layer(0.2 * MM, 4.3, 0.02)])
+    shapes = {1: [(-L / 2, -w / 2, L / 2, w / 2)]}
+    vp = lambda: [ViaPort(Via(-L / 2, 0.0, 0, 1, 0.1 * MM), 50.0, 'P1'), ViaPort(Via(L / 2, 0.0, 0, 1, 0.1 * MM), 50.0, 'P2')]
+    kw = dict(dmax=0.3 * MM, ncross=4, w_ref={1: w}, precision='double', verbose=False)
+    Zel = 37.0
+    with_el = Simulation(st, shapes, vp(), lumped=[SeriesLumped(1, (0.0, 0.0), (1.0, 0.0), w, Zel)], **kw)
+    three = Simulation(st, shapes, vp() + [GapPort(1, (0.0, 0.0), (1.0, 0.0), w, 50.0, 'G')], **kw)
+    I3 = np.eye(3)
+    for f in (2e9, 7e9):
+        S3 = three.sparams(f)
+        Z3 = 50.0 * (I3 + S3) @ np.linalg.inv(I3 - S3)
+        Z2 = Z3[:2, :2] - np.outer(Z3[:2, 2], Z3[2, :2]) / (Z3[2, 2] + Zel)
+        S2 = (Z2 - 50.0 * np.eye(2)) @ np.linalg.inv(Z2 + 50.0 * np.eye(2))
+        Se = with_el.sparams(f)
+        assert np.abs(Se - S2).max() <= 1e-9, (f, np.abs(Se - S2).max())

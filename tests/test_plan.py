@@ -24,7 +24,10 @@ def test_device_footprint_and_fit():
     p = memory_plan(n, gpu=('hip', 288e9))
     assert p['gpu'] == 24.0 * n * n + 4 * 8.0 * n * n + WORK_BYTES and p['fits']        # 90 GB of 288
     assert not memory_plan(n, gpu=('hip', 16e9))['fits']                                # a 16 GB GPU: no
-    assert memory_plan(n, gpu=('hip', 16e9), keep=0)['gpu'] == 24.0 * n * n + WORK_BYTES
+    assert memory_plan(n, gpu=('hip', 60e9), keep=0)['gpu'] == 24.0 * n * n + WORK_BYTES
+    # where the FP64 Z does not fit but the complex64 one does: the lean path (the refinement without Z)
+    p = memory_plan(n, gpu=('hip', 16e9), keep=0)
+    assert p['path'] == 'gpu-stream' and p['fits'] and p['gpu'] == 8.0 * n * n + WORK_BYTES
     assert not memory_plan(1000, gpu=None)['fits']                                     # no GPU: nothing fits
     # 20 000 unknowns, the serving configuration's largest board: 24 n^2 = 9.6 GB and four kept factors, 22.4 GB
     assert abs(device_bytes(20000, 'mixed') - 22.4e9 - WORK_BYTES) < 1

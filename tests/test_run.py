@@ -49,6 +49,26 @@ def test_solve_band_linear_and_keep():
     assert list(out_refs(sim, 25.0)) == [25.0, 25.0]
 
 
-def test_solve_band_adaptive_is_refused():
-    with pytest.raises(ValueError):
-        solve_band(_Sim(), 1e9, 3e9, log=lambda *_: None)
+class _Log:
+    def __init__(self):
+        self.notes = []
+
+    def note(self, msg):
+        self.notes.append(msg)
+
+
+def test_solve_band_adaptive():
+    """The default sweep (adaptive): the stand-in's S is linear in f, so the model is exact after a few solves;
+    it converges well before the cap, its model equals the truth on the requested grid, keep_at is among the
+    solved frequencies, and the history reaches on_step."""
+    sim = _Sim()
+    sim.log = _Log()
+    steps = []
+    f, S, info = solve_band(sim, 1e9, 3e9, points=41, log=lambda *_: None, keep_at=[1.7e9], on_step=steps.append,
+                            max_solves=12)
+    assert len(f) == 41 and S.shape == (41, 2, 2)
+    truth = np.array([_Sim().sparams(fi) for fi in f])
+    assert np.abs(S - truth).max() < 1e-9
+    assert info['solves'] < 12 and info['afs'][-1]['converged'] and info['afs_tol'] == 2e-3
+    assert 1.7e9 in info['solved'] and list(sim.kept) == [1.7e9]
+    assert steps and steps[-1] is info['afs'][-1] and not sim.log.notes
