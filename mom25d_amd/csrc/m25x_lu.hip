@@ -632,9 +632,10 @@ template <> struct TrsvMaxRhs<double2> { static constexpr int NR = 4; };
 
 // Unit-lower (FWD) or unit-upper substitution of NW right-hand sides, lane i row i, with the block's operator Ms
 // in LDS (rows pre-scaled to a unit diagonal; the diagonal, the other triangle and the padding zero): step k
-// broadcasts y_k and every lane subtracts its multiple, the zeros leaving the finished rows alone.
-template <bool FWD, int NW, typename T>
-__device__ __attribute__((always_inline)) void subst(T (&t)[NW], const T (*Ms)[SB + 1], int lane)
+// broadcasts y_k and every lane subtracts its multiple, the zeros leaving the finished rows alone. After each 8
+// steps the 8 rows they finished (lanes [lo, lo + 8)) are final: pub(lo) hands them on while the rest go on.
+template <bool FWD, int NW, typename T, class Pub>
+__device__ __attribute__((always_inline)) void subst(T (&t)[NW], const T (*Ms)[SB + 1], int lane, Pub &pub)
 {
 #pragma unroll
     for (int s0 = 0; s0 < SB; s0 += 8) {
@@ -647,6 +648,7 @@ __device__ __attribute__((always_inline)) void subst(T (&t)[NW], const T (*Ms)[S
 #pragma unroll
             for (int w = 0; w < NW; ++w) t[w] = cfms(t[w], mk[j], readlane_c(t[w], k));
         }
+        pub(FWD ? s0 : SB - 8 - s0);
     }
 }
 
@@ -657,10 +659,13 @@ __device__ __attribute__((always_inline)) void subst(T (&t)[NW], const T (*Ms)[S
 //  - it then meets the earlier blocks a in order: thread (row lane, columns 8 wv + u of block a) holds its
 //    entries of Op_qa, loaded TrsvDepth blocks ahead, and adds Op_qa y_a to its partial sums once y_a is out;
 //  - after the last one (a = q - 1) the partial sums meet in LDS, waves 0-3 form the block's right-hand side
-//    (wave w takes right-hand sides w and, for NR = 8, w + 4; lane i row i), substitute, and publish y_q.
-// y travels as self-marking values (put / take) in dst: a reader spins on the value itself, tightly when the block
-// is one of the two before its own (the chain waits on it), with growing naps otherwise. Workgroup q waits only on
-// workgroups before it, which start no later, so the launch makes progress with any number of them resident.
+//    (wave w takes right-hand sides w and, for NR = 8, w + 4; lane i row i), substitute, and publish y_q eight
+//    rows at a time as the substitution finishes them: the next workgroup's wave that takes those rows starts its
+//    product with them while the substitution goes on, so the chain's hand-off overlaps the substitution.
+// y travels as self-marking values (put / take) in dst: a reader spins on the values themselves, every lane on its
+// own when the block is one of the two before its own (the chain waits on it), one lane with growing naps first
+// otherwise. Workgroup q waits only on workgroups before it, which start no later, so the launch makes progress
+// with any number of them resident.
 //
 // rdg: 1 / U_ii (read for the kinds with U only: L is unit). src: n x nrhs (leading dimension lds); dst: n x nrhs
 // (leading dimension nrhs) filled with UNSET words beforehand. abort: raised if a wait exceeds its bound (the result
@@ -697,17 +702,18 @@ __device__ __forceinline__ void trsv_body(const T *LU, long lda, long n, const T
         md[u] = *opp(gi, gk);
         mr[u] = rdg[gi];                     // read either way: a select of values, not of addresses
     }
-    // the solver waves' right-hand sides and 1 / U_ii of the lane's row (rows past n: zeroed where used)
-    T bx[NW], rdl;
+    // the solver waves' right-hand sides and 1 / U_ii of the lane's row (rows past n: zeroed where used), parked in
+    // LDS until the substitution: held in registers across the loop they are spilled to scratch for complex128, and
+    // reloading them would put a memory round trip into every step of the chain
+    __shared__ T bxs[NR][SB], rds[SB];
     int rr[NW];
-    {
+#pragma unroll
+    for (int w = 0; w < NW; ++w) rr[w] = (wv & 3) + 4 * w;
+    if (wv < 4) {
         const long gi = r0 + std::min(lane, m - 1);
 #pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            rr[w] = (wv & 3) + 4 * w;
-            bx[w] = src[gi * lds + std::min(rr[w], nrhs - 1)];
-        }
-        rdl = csel(unit, one, rdg[gi]);
+        for (int w = 0; w < NW; ++w) bxs[rr[w]][lane] = src[gi * lds + std::min(rr[w], nrhs - 1)];
+        if (wv == 0) rds[lane] = csel(unit, one, rdg[gi]);
     }
 
     // the earlier blocks' entries: thread (row lane, columns 8 wv + u of block a); a row past the matrix reads the
@@ -751,24 +757,20 @@ __device__ __forceinline__ void trsv_body(const T *LU, long lda, long n, const T
         const int a = order(p);
         const long g0 = (long)a * SB + wv * TE, gk = g0 + lu_;
         const bool need = lane < TE * NR && lr < nrhs && gk < n;
-        if (lane == 0 && g0 < n) {           // one lane watches the slice's first value (a slice past n needs none)
-            const T *w0 = dst + g0 * nrhs;
-            const bool near = q - p <= 2;
+        if (q - p > 2 && lane == 0 && g0 < n) {   // far from the chain: one lane watches the slice's first value
+            const T *w0 = dst + g0 * nrhs;         // with growing naps first (a slice past n needs none)
             T v;
             long spins = 0;
             int nap = 1;
             while (!take(w0, v)) {
-                if (near) __builtin_amdgcn_s_sleep(1);
-                else {
-                    for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(2);
-                    nap = nap < 8 ? 2 * nap : 8;
-                }
+                for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(2);
+                nap = nap < 8 ? 2 * nap : 8;
                 if (++spins > (1L << 22)) {
                     __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
             }
-        }
+        }                                          // near it, every lane polls its own value from the start
         T yv = zero;
         bool ok = !need || take(dst + gk * nrhs + lr, yv);
         long spins = 0;
@@ -808,13 +810,17 @@ __device__ __forceinline__ void trsv_body(const T *LU, long lda, long n, const T
         T v = zero;
 #pragma unroll
         for (int sl = 0; sl < NWV; ++sl) v = tadd(v, red[sl][rr[w]][lane]);
-        t[w] = csel(inrow && rr[w] < nrhs, tmul(tsub(bx[w], v), rdl), zero);
+        t[w] = csel(inrow && rr[w] < nrhs, tmul(tsub(bxs[rr[w]][lane], v), rds[lane]), zero);
     }
-    if (fwd) subst<true, NW>(t, Ms, lane);
-    else subst<false, NW>(t, Ms, lane);
+    auto pub = [&](int lo) __attribute__((always_inline)) {
+        if (lane >= lo && lane < lo + 8 && inrow) {
 #pragma unroll
-    for (int w = 0; w < NW; ++w)
-        if (rr[w] < nrhs && inrow) put(dst + (r0 + lane) * nrhs + rr[w], t[w]);
+            for (int w = 0; w < NW; ++w)
+                if (rr[w] < nrhs) put(dst + (r0 + lane) * nrhs + rr[w], t[w]);
+        }
+    };
+    if (fwd) subst<true, NW>(t, Ms, lane, pub);
+    else subst<false, NW>(t, Ms, lane, pub);
 }
 
 template <typename T, int NR>
