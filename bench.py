@@ -178,9 +178,9 @@ def tslu_steps(n, leaf=256, arity=4):
 
 
 def roofline_only(sim, freqs):
-    """The kernels whose rooflines the benchmark reports, alone, for rocprofv3 runs: the complex64 LU of the board
-    at 2 GHz (k_tslu, k_gemm, ...), ten m25x_getrs calls with its four right-hand sides, ten far fills, and a 1 GiB
-    device-to-device copy whose counter readings give the scale of FETCH_SIZE / WRITE_SIZE."""
+    """The kernels whose rooflines the benchmark reports, alone, for rocprofv3 runs: the complex64 and complex128 LUs
+    of the board at 2 GHz (k_tslu, k_gemm, ...), ten m25x_getrs calls with each and the four right-hand sides, ten
+    far fills, and a 1 GiB device-to-device copy whose counter readings give the scale of FETCH_SIZE / WRITE_SIZE."""
     import torch
     from mom25d_amd import device as dv
     f = freqs[5]
@@ -195,13 +195,17 @@ def roofline_only(sim, freqs):
     for _ in range(10):
         dv.getrs(F, B)
     del F, Z64
+    F = dv.getrf(Z)                                   # complex128, in place of the equilibrated Z
+    for _ in range(10):
+        dv.getrs(F, B)
+    del F, Z
     fo = sim.fast
     Zf = torch.empty((sim.Ne, sim.Ne), dtype=dv.c128, device=dv.device())
     for _ in range(10):
         dv.fill_far(Zf, fo.mesh, fo.tabs, fo.w)
     rec = dv.timing_records()
     dv.timing(())
-    src = torch.empty(1 << 30, dtype=torch.uint8, device=Z.device)
+    src = torch.empty(1 << 30, dtype=torch.uint8, device=dv.device())
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
