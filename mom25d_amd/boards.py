@@ -6,9 +6,10 @@
                         Manhattan traces (width U(0.15, 0.5) mm) on interfaces 2 and 4, 6 vias (r 0.15 mm) joining
                         them, 4 via ports to ground
   coupled_lines(seed)   C3: a 2-layer board with two parallel 0.3 mm strips 0.2 mm apart, 40 mm long, 4 ports
-  trace_board(seed)     C5: a one-layer board of rng.integers(2, 17) traces from the C1 generator
-  uneven_board(seed)    C4: the same kind of board (narrower, shorter traces) meshed to a drawn size N ~ U(500, 6000)
-                        (the mesh density, and where needed the cells across a trace, solved for)
+  trace_board(seed)     a one-layer board of rng.integers(2, 17) traces from the C1 generator
+  uneven_board(seed)    C4 and C5: the same kind of board (narrower, shorter traces) meshed to a drawn size N ~
+                        U(n_range) (the mesh density, and where needed the cells across a trace, solved for): C4
+                        U(500, 6000), C5 U(2000, 20000)
 Each returns (stack, shapes, ports, vias, kwargs) for Simulation(stack, shapes, ports, vias, **kwargs); the seeds
 and the drawn parameters are deterministic (numpy default_rng).
 """
@@ -106,7 +107,9 @@ def uneven_board(seed, n_range=(500, 6000), ncross=3):
     """A one-layer FR4 board (the C1 stack) of rng.integers(2, 17) traces of width U(0.3, 0.6) mm and 1-3 segments
     of U(1, 2.5) mm, a via port to ground at the first four traces' starts, meshed to a size drawn from
     U(n_range): the mesh density dmax is solved for (the unknowns grow as 1 / dmax along the traces), between 0.02
-    and 2 mm, with fewer cells across the traces where even the coarsest mesh has too many unknowns. Returns the usual (stack, shapes, ports, vias, kwargs) with kwargs['target_n'] the drawn size."""
+    and 2 mm, with fewer cells across the traces where even the coarsest mesh has too many unknowns, and coarsened
+    until the size is within the range's top. Returns the usual (stack, shapes, ports, vias, kwargs) with
+    kwargs['target_n'] the drawn size."""
     from .sim import Simulation
     rng = np.random.default_rng(seed)
     st = LayerStack([Layer(0.8 * MM, 4.4, 0.02)])
@@ -132,4 +135,8 @@ def uneven_board(seed, n_range=(500, 6000), ncross=3):
             dmax = float(np.clip(dmax * size(dmax, nc) / target, 0.02 * MM, 2.0 * MM))
         if size(dmax, nc) <= 1.25 * target:
             break
+    for _ in range(60):                         # never above the range's top
+        if dmax >= 2.0 * MM or size(dmax, nc) <= n_range[1]:
+            break
+        dmax = min(2.0 * MM, 1.03 * dmax)
     return st, shapes, ports, [], dict(dmax=dmax, ncross=nc, w_ref={1: 0.6 * MM}, target_n=int(target))
