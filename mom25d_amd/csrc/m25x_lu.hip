@@ -633,20 +633,26 @@ template <> struct TrsvMaxRhs<double2> { static constexpr int NR = 4; };
 // Unit-lower (FWD) or unit-upper substitution of NW right-hand sides, lane i row i, with the block's operator Ms
 // in LDS (rows pre-scaled to a unit diagonal; the diagonal, the other triangle and the padding zero): step k
 // broadcasts y_k and every lane subtracts its multiple, the zeros leaving the finished rows alone. After each 8
-// steps the 8 rows they finished (lanes [lo, lo + 8)) are final: pub(lo) hands them on while the rest go on.
+// steps the 8 rows they finished (lanes [lo, lo + 8)) are final: pub(lo) hands them on while the rest go on. The
+// row's operator entries are read from LDS one group of 8 steps ahead.
 template <bool FWD, int NW, typename T, class Pub>
 __device__ __attribute__((always_inline)) void subst(T (&t)[NW], const T (*Ms)[SB + 1], int lane, Pub &pub)
 {
+    T mk[2][8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) mk[0][j] = Ms[lane][FWD ? j : SB - 1 - j];
 #pragma unroll
     for (int s0 = 0; s0 < SB; s0 += 8) {
-        T mk[8];
+        const int g = (s0 / 8) & 1;
+        if (s0 + 8 < SB) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) mk[j] = Ms[lane][FWD ? s0 + j : SB - 1 - s0 - j];
+            for (int j = 0; j < 8; ++j) mk[g ^ 1][j] = Ms[lane][FWD ? s0 + 8 + j : SB - 9 - s0 - j];
+        }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int k = FWD ? s0 + j : SB - 1 - s0 - j;
 #pragma unroll
-            for (int w = 0; w < NW; ++w) t[w] = cfms(t[w], mk[j], readlane_c(t[w], k));
+            for (int w = 0; w < NW; ++w) t[w] = cfms(t[w], mk[g][j], readlane_c(t[w], k));
         }
         pub(FWD ? s0 : SB - 8 - s0);
     }
@@ -807,9 +813,14 @@ __device__ __forceinline__ void trsv_body(const T *LU, long lda, long n, const T
     T t[NW];
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
-        T v = zero;
+        T h[NWV];                            // the 8 waves' partial sums, added as a tree (depth 3)
 #pragma unroll
-        for (int sl = 0; sl < NWV; ++sl) v = tadd(v, red[sl][rr[w]][lane]);
+        for (int sl = 0; sl < NWV; ++sl) h[sl] = red[sl][rr[w]][lane];
+#pragma unroll
+        for (int st = 1; st < NWV; st *= 2)
+#pragma unroll
+            for (int sl = 0; sl + st < NWV; sl += 2 * st) h[sl] = tadd(h[sl], h[sl + st]);
+        const T v = h[0];
         t[w] = csel(inrow && rr[w] < nrhs, tmul(tsub(bxs[rr[w]][lane], v), rds[lane]), zero);
     }
     auto pub = [&](int lo) __attribute__((always_inline)) {
