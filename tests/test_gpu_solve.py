@@ -109,6 +109,30 @@ def test_seeded_8k_board_double_vs_mixed(gpu):
     assert np.abs(Sr - sd.sparams(f2)).max() <= 1e-4
 
 
+def test_rounded_double_factor_preconditions_as_the_double_one(gpu):
+    """C2's low-frequency end: the complex64 factor does not converge at 1 GHz, so the point is solved with a
+    complex128 factor, which is kept (by default rounded to complex64) and preconditions 1.15 GHz. Kept rounded and
+    kept as complex128, the two runs give S within the mixed bound of the double-precision answer and within 2e-5 of
+    each other, in GMRES iterations that differ by at most two a port."""
+    from mom25d_amd import device as dv, fast
+    sd = make(boards.seeded_board, 'double')
+    runs = {}
+    for keep in ('c64', 'c128'):
+        fast.KEEP_DOUBLE = keep
+        try:
+            sm = make(boards.seeded_board, 'mixed')
+            S = [sm.sparams(f) for f in (1.0e9, 1.15e9)]
+            runs[keep] = (S, list(sm.fast.iters), [F.dtype for _, F, _, _ in sm.fast._kept])
+        finally:
+            fast.KEEP_DOUBLE = 'c64'
+    assert runs['c64'][2] == [dv.c64] and runs['c128'][2] == [dv.c128]
+    for f, a, b in zip((1.0e9, 1.15e9), runs['c64'][0], runs['c128'][0]):
+        Sd = sd.sparams(f)
+        assert np.abs(a - Sd).max() <= 1e-4 and np.abs(b - Sd).max() <= 1e-4, f
+        assert np.abs(a - b).max() <= 2e-5, f
+    assert max(abs(x - y) for x, y in zip(runs['c64'][1], runs['c128'][1])) <= 2
+
+
 def test_matrix_free_operator_and_lean_path(gpu):
     """matvec_exact (far field triangle-aggregated, near correction as a sparse matrix, via rows and columns) equals
     the dense FP64 Z times X to 1e-12 of each column's largest value on the coupled-line board (N ~ 3k, 4 via
