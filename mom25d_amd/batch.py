@@ -3,7 +3,8 @@
 No counterpart in the reference, which solves one point at a time (src/mom25d/run.py:55-64 calls
 Simulation.sparams per frequency; src/mom25d/fast.py:704-850 fills and factors one Z). A batch:
 
-  fill      every point's Z (FP64) on the device, equilibrated (D Z D), and its complex64 copy
+  fill      every point's Z (FP64) on the device, equilibrated (D Z D), and its complex64 copy: first each point's
+            host-side parts (tables, near moments, via blocks), then every Z queued with nothing waiting
   factor    all complex64 copies in one sequence of launches (m25x_getrf_batched: each panel step one launch per
             kernel for all points), optionally captured once in a HIP graph and replayed for the next batch of the
             same sizes
@@ -80,12 +81,14 @@ class SweepBatch:
         t0 = time.perf_counter()
         pts = self.points
         Zs, Z64s, ds, Bs, ks = [], [], [], [], []
-        for sim, f in pts:
-            Z = sim.fast.fill(f)
+        # every point's host-side parts first (tables, moments, via blocks: these wait for the device), then every
+        # Z, equilibration and copy queued with nothing waiting, so the fills run back to back on the device
+        prep = [(sim.fast.prepare(f), torch.from_numpy(sim.excitation()).to(dv.device())) for sim, f in pts]
+        for (sim, f), (st, E) in zip(pts, prep):
+            Z = sim.fast.assemble(st)
             d = dv.diag_scales(Z)
             Z64 = torch.empty(Z.shape, dtype=dv.c64, device=Z.device)
             dv.scale_c64(Z, d, Z64)                          # Z <- D Z D (FP64, kept), Z64 its complex64 copy
-            E = torch.from_numpy(sim.excitation()).to(Z.device)
             Zs.append(Z); Z64s.append(Z64); ds.append(d); Bs.append(d[:, None] * E); ks.append(E.shape[1])
         t1 = time.perf_counter()
         if self.graph:

@@ -71,41 +71,59 @@ class FastOperator:
         self.sim.green_tables(f)
         self.tabs = self.sim.dev_tables
 
-    def fill(self, f, dtype=dv.c128):
-        """Z (device, N x N, not equilibrated) at f: complex128, or complex64 for the lean path (each entry computed
-        in FP64 and rounded once; the near correction added in FP32). The parts the matrix-free operator needs are
-        kept for this frequency: the near entries' values, the via blocks B and C and the lumped entries."""
+    def prepare(self, f):
+        """The per-frequency parts of Z that involve the host, computed before any N x N work is queued: the
+        Green's tables, the near pairs' moments, the via blocks B and C (their small dense parts are assembled on
+        the host, which waits for the device there) and the lumped entries, uploaded. Returns the point's state for
+        assemble(); holding it keeps its tables alive, so many points can be prepared before any is assembled."""
         sim, w = self.sim, 2 * np.pi * f
-        Ne, Nv = sim.Ne, sim.Nv
-        N = Ne + Nv
         self.setup(f)
-        dev = dv.device()
-        with sim.log.timer('direct fill', f"N={N}, {'complex128' if dtype == dv.c128 else 'complex64'}, GPU"):
-            Z = torch.empty((N, N), dtype=dtype, device=dev)
-            dv.fill_far(Z, self.mesh, self.tabs, w)
-        with sim.log.timer('near-field correction'):
-            M, qc = dv.near_moments(self.mesh, self.tabs, self.plan, w)
-            dv.near_scatter(Z, self.mesh, self.plan, M, w)
-            self._op = dict(f=f, w=w, vals=dv.near_values(self.mesh, self.plan, M, w) if dtype == dv.c64 else None,
-                            B=None, C=None, lumped=None, M=M)
-        if Nv:
-            with sim.log.timer('via kernels', f"{Nv} chains"):
-                QF = dv.fan_q(self.mesh, self.tabs, sim.F)
+        tabs, dev = self.tabs, dv.device()
+        with sim.log.timer('near-field moments'):
+            M, qc = dv.near_moments(self.mesh, tabs, self.plan, w)
+        B = C = lum = None
+        if sim.Nv:
+            with sim.log.timer('via kernels', f"{sim.Nv} chains"):
+                QF = dv.fan_q(self.mesh, tabs, sim.F)
                 if self._fan is not None:
                     sel, rows, cols = self._fan
                     QF[rows, cols] += qc[sel]
                 B, C = sim._via_blocks(f, QF)
-                Z[:Ne, Ne:] = B
-                Z[Ne:, :Ne] = B.T
-                Z[Ne:, Ne:] = C
-                self._op.update(B=B, C=C)
-        lum = sim.lumped_entries(f)
-        if lum is not None:
-            r, c, v = (torch.from_numpy(x).to(dev) for x in lum)
-            v = v.to(dv.c128)
+        le = sim.lumped_entries(f)
+        if le is not None:
+            r, c, v = (torch.from_numpy(x).to(dev) for x in le)
+            lum = (r, c, v.to(dv.c128))
+        return dict(f=f, w=w, tabs=tabs, M=M, B=B, C=C, lumped=lum)
+
+    def assemble(self, st, dtype=dv.c128):
+        """Z (device, N x N, not equilibrated) of a prepared point: complex128, or complex64 for the lean path
+        (each entry computed in FP64 and rounded once; the near correction added in FP32). Only device work is
+        queued (nothing waits for the device). The parts the matrix-free operator needs are kept for this
+        frequency: the near entries' values, the via blocks B and C and the lumped entries."""
+        sim, w = self.sim, st['w']
+        Ne, Nv = sim.Ne, sim.Nv
+        N = Ne + Nv
+        self.f, self.w, self.tabs = st['f'], w, st['tabs']
+        with sim.log.timer('direct fill', f"N={N}, {'complex128' if dtype == dv.c128 else 'complex64'}, GPU"):
+            Z = torch.empty((N, N), dtype=dtype, device=dv.device())
+            dv.fill_far(Z, self.mesh, self.tabs, w)
+        with sim.log.timer('near-field correction'):
+            dv.near_scatter(Z, self.mesh, self.plan, st['M'], w)
+        self._op = dict(f=st['f'], w=w, vals=dv.near_values(self.mesh, self.plan, st['M'], w) if dtype == dv.c64 else None,
+                        B=st['B'], C=st['C'], lumped=st['lumped'], M=st['M'])
+        if Nv:
+            B, C = st['B'], st['C']
+            Z[:Ne, Ne:] = B
+            Z[Ne:, :Ne] = B.T
+            Z[Ne:, Ne:] = C
+        if st['lumped'] is not None:
+            r, c, v = st['lumped']
             Z.index_put_((r, c), v.to(dtype), accumulate=True)
-            self._op['lumped'] = (r, c, v)
         return Z
+
+    def fill(self, f, dtype=dv.c128):
+        """Z (device, N x N, not equilibrated) at f: prepare(f) then assemble (see both)."""
+        return self.assemble(self.prepare(f), dtype)
 
     def matvec_exact(self, X):
         """Z X in FP64 without Z (the reference's matvec_exact, fast.py:571-586) at the frequency of the last fill:

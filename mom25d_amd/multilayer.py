@@ -241,13 +241,14 @@ _SPLINES = {}
 
 def spline_operator(rho_max, nrho, nfine):
     """The (nfine x nrho) matrix S with S @ y = CubicSpline(rho_max u^2, y)(uniform nfine grid) (not-a-knot, the
-    reference's Table1D, via.py:111-123): the spline is linear in its data, so its action is fixed by the grid.
-    Built once per grid on the host, kept on the device."""
-    key = (float(rho_max), int(nrho), int(nfine), torch.cuda.current_device())
+    reference's Table1D, via.py:111-123): the spline is linear in its data, so its action is fixed by the grid, and
+    a cubic spline is unchanged by a scaling of its abscissae, so one S (built on rho_max = 1) serves every rho_max.
+    Built once per (nrho, nfine) on the host, kept on the device."""
+    key = (int(nrho), int(nfine), torch.cuda.current_device())
     S = _SPLINES.get(key)
     if S is None:
-        rho = rho_max * np.linspace(0, 1, nrho) ** 2
-        fine = np.linspace(0, rho_max, nfine)
+        rho = np.linspace(0, 1, nrho) ** 2
+        fine = np.linspace(0, 1, nfine)
         S = torch.from_numpy(np.ascontiguousarray(CubicSpline(rho, np.eye(nrho))(fine))).to(dv.device())
         _SPLINES[key] = S
     return S

@@ -30,9 +30,11 @@ from .batch import SweepBatch
 
 def request_bytes(sim):
     """Device bytes one request holds while it is solved: the FP64 Z (16 N^2), the complex64 factor (8 N^2), its LU
-    and solve workspaces and right-hand sides (bounded by 2 KiB per unknown)."""
+    and solve workspaces and right-hand sides (bounded by 2 KiB per unknown), and its near pairs' moments (128 B a
+    pair, held from the batch's preparation to its fill)."""
     n = sim.Ne + sim.Nv
-    return 24.0 * n * n + 2048.0 * n
+    plan = getattr(getattr(sim, 'fast', None), 'plan', None)
+    return 24.0 * n * n + 2048.0 * n + 128.0 * (plan.n if plan is not None else 0)
 
 
 class Server:

@@ -112,19 +112,60 @@ class Via:
 
 
 class LinePort:
-    """A straight line across a feed on one interface: centre, unit direction d into the device, t = d turned by 90
-    degrees, width (geo.py:24-60)."""
-    kind = 'line'
+    """A wave port on a straight feed (geo.py:24-60): a delta-gap source across the feed at `center` (interface
+    `layer`, `width` wide), `direction` pointing into the device along the feed, the reference plane ref_dist
+    beyond the gap. The feed's current is sampled on cuts across it every `step` (dmax if None) from s0 (2 step if
+    None) to s1 (ref_dist if None) from the gap, and fitted by a forward and a backward wave of one propagation
+    constant (fitted, or gamma_fn(f) when set); zc is the reference impedance. It is also the geometry of a gap
+    port's or a series element's line (t = d turned by 90 degrees)."""
+    kind = 'wave'
 
-    def __init__(self, layer, center, direction, width, ref_dist=0.0, name=''):
+    def __init__(self, layer, center, direction, width, ref_dist=0.0, s0=None, step=None, zc=50.0, name='', s1=None):
         d = np.asarray(direction, float)
         self.d = d / np.linalg.norm(d)
         self.t = np.array([-self.d[1], self.d[0]])
         self.layer, self.c, self.w, self.ref, self.name = layer, np.asarray(center, float), width, ref_dist, name
+        self.s0, self.step, self.s1, self.zc = s0, step, s1, float(zc)
+        self.zc_fn = self.gamma_fn = None
 
     @property
     def axis_aligned(self):
         return bool(np.isclose(self.d, 0.0, atol=1e-12).any())
+
+    def line(self, s=0.0):
+        """End points of the cut across the feed at distance s from the gap."""
+        p = self.c + self.d * s
+        return p - self.t * self.w / 2, p + self.t * self.w / 2
+
+    def cuts(self, dmax):
+        """The sample positions along the feed: every step back from s1 to no closer than s0."""
+        step = self.step or dmax
+        s0 = 2 * step if self.s0 is None else self.s0
+        s1 = self.ref if self.s1 is None else self.s1
+        n = int(np.floor((s1 - s0) / step + 1e-9))
+        return s1 - step * np.arange(n, -1, -1)
+
+
+FIT_SPAN = 0.4          # rad: the phase a wave port's cuts must span for a fitted propagation constant to be trusted
+
+
+def wave_gamma(Ik, step):
+    """The propagation constant of samples Ik (ncut x ncol) of I(s) = a e^(-gamma s) + b e^(gamma s) at uniform
+    spacing step: any such sequence obeys I[k-1] + I[k+1] = 2 cosh(gamma step) I[k], so cosh(gamma step) is the
+    least-squares solution of that relation over every interior cut and column; gamma with Re, Im >= 0."""
+    Ik = np.asarray(Ik, complex).reshape(len(Ik), -1)
+    lhs, rhs = 2 * Ik[1:-1].ravel(), (Ik[:-2] + Ik[2:]).ravel()
+    ch = np.vdot(lhs, rhs) / np.vdot(lhs, lhs)
+    g = np.arccosh(complex(ch)) / step
+    return complex(abs(g.real), abs(g.imag))
+
+
+def wave_split(Ik, s, gamma):
+    """(a, b) per column with Ik[:, j] ~ a_j e^(-gamma s) + b_j e^(gamma s), least squares over the cuts s."""
+    s = np.asarray(s, float)
+    M = np.stack([np.exp(-gamma * s), np.exp(gamma * s)], 1)
+    coef = np.linalg.lstsq(M, np.asarray(Ik, complex).reshape(len(s), -1), rcond=None)[0]
+    return coef[0], coef[1]
 
 
 class GapPort:
@@ -246,6 +287,7 @@ class Simulation:
         self.gpu, self.reuse, self.stream = gpu, 'auto', 'auto'
         self.dmax, self.ncross, self.grade, self.near_fac = dmax, ncross, grade, near_fac
         self.ports, self.lumped, self.vias = list(ports), list(lumped), list(vias)
+        self.samples = [p.cuts(dmax) if isinstance(p, LinePort) else None for p in self.ports]
         for p in self.ports:
             if isinstance(p, ViaPort) and p.via not in self.vias:
                 self.vias.append(p.via)
@@ -266,7 +308,10 @@ class Simulation:
         self.P3, self.W3, self.P6, self.W6 = self.q['P3'], self.q['W3'], self.q['P6'], self.q['W6']
         span = self.rwg.nodes.max(0) - self.rwg.nodes.min(0)
         self.rho_max = 1.05 * np.hypot(*span) + 1e-4
-        self.gap = [self._edges_on(p.lp, f"port {p.name}") if isinstance(p, GapPort) else None for p in self.ports]
+        self.gap = [self._edges_on(p.lp, f"port {p.name}") if isinstance(p, GapPort) else
+                    self._edges_on(p, f"port {p.name}") if isinstance(p, LinePort) else None for p in self.ports]
+        self.cuts = [None if s is None else [self._edges_on(p, f"port {p.name}: the sample cut at {x * 1e3:.3f} mm", x)
+                                             for x in s] for p, s in zip(self.ports, self.samples)]
         self.lumped_edges = [self._edges_on(el.lp, f"lumped element {el.name}") for el in self.lumped]
         self._gram = None
         n = self.Ne + self.Nv
@@ -298,13 +343,14 @@ class Simulation:
         base = 0
         for k in self.layers:
             across = {0: [], 1: []}                 # ports and elements: a line across x (a constant x) or y
-            feeds = [p.lp for p in self.ports if isinstance(p, GapPort) and p.layer == k] + \
-                [el.lp for el in self.lumped if el.layer == k]
-            for lp in feeds:
+            feeds = [(p.lp, [0.0]) for p in self.ports if isinstance(p, GapPort) and p.layer == k] + \
+                [(p, [0.0] + list(sp)) for p, sp in zip(self.ports, self.samples) if isinstance(p, LinePort)
+                 and p.layer == k] + [(el.lp, [0.0]) for el in self.lumped if el.layer == k]
+            for lp, at in feeds:
                 if not lp.axis_aligned:
-                    raise ValueError("gap ports and elements must be axis-aligned on a rectangle board")
+                    raise ValueError("ports and elements must be axis-aligned on a rectangle board")
                 axis = 0 if abs(lp.d[0]) > 0.5 else 1
-                across[axis].append(lp.c[axis])
+                across[axis] += [lp.c[axis] + lp.d[axis] * x for x in at]
             here = [(i, v) for i, v in enumerate(self.vias) if k in self.via_attach[i]]
             dm = dmax_layer.get(k, self.dmax)
             ref = w_ref.get(k) if isinstance(w_ref, dict) else w_ref
@@ -340,11 +386,11 @@ class Simulation:
                 if tri is not None:
                     self.dF[[row[t] for t in tri], n] += sign / r.area[tri].sum()
 
-    def _edges_on(self, lp, what, tol=1e-8):
-        """(edges, signs) of the RWG edges lying on a feed line within its width, on its interface: the sign is
-        that of the edge's current direction along the feed (sim.py:697-728)."""
+    def _edges_on(self, lp, what, s=0.0, tol=1e-8):
+        """(edges, signs) of the RWG edges lying on a feed's cut at distance s from its gap, within its width, on
+        its interface: the sign is that of the edge's current direction along the feed (sim.py:697-728)."""
         r = self.rwg
-        rel = r.ends - lp.c[None, None, :]                     # (Ne, 2 ends, 2)
+        rel = r.ends - (lp.c + lp.d * s)[None, None, :]       # (Ne, 2 ends, 2)
         along, across = rel @ lp.d, rel @ lp.t
         on = (np.abs(along) < tol).all(1) & (np.abs(across) <= lp.w / 2 + tol).all(1) & (self.edge_layer == lp.layer)
         idx = np.flatnonzero(on)
@@ -405,18 +451,18 @@ class Simulation:
         B = torch.zeros((self.Ne, self.Nv), dtype=dv.c128, device=dev)
         C = np.zeros((self.Nv, self.Nv), complex)
         vk = ViaKernels(f, self.st, [(lo, hi) for _, lo, hi in self.chains], self.layers, self.rho_max)
-        geo, div = self._via_geometry(), self._div_ops()
+        geo, div, vd = self._via_geometry(), self._div_ops(), self._via_device()
         if len(self.F):
-            pot = QF @ torch.from_numpy(self.dF).to(dev, dv.c128)            # (Nt, Nv) the end charges' potential
+            pot = QF @ vd['dF']                                               # (Nt, Nv) the end charges' potential
             B += div(pot) / jw
-            C += (self.dF.T @ pot[torch.from_numpy(self.F).to(dev)].cpu().numpy()) / jw
+            C += (self.dF.T @ pot[vd['F']].cpu().numpy()) / jw
         for span, ns in geo['by_span'].items():
             col = torch.zeros((Nt, len(ns)), dtype=dv.c128, device=dev)
             for k in self.layers:
-                idx, D = self.tri_idx[k], vk.D[(k, span)]
-                col[torch.from_numpy(idx).to(dev)] = dv.radial_table_sum(
-                    self.P6[idx, :, 0], self.P6[idx, :, 1], self.W6[idx], geo['x'][ns], geo['y'][ns], D.v, D._dr)
-            B[:, torch.from_numpy(ns).to(dev)] += div(col)
+                idx, px, py, w6 = vd['tri'][k]
+                D = vk.D[(k, span)]
+                col[idx] = dv.radial_table_sum(px, py, w6, vd['x'][span], vd['y'][span], D.v, D._dr)
+            B[:, vd['ns'][span]] += div(col)
         for (sm, sn), (m, n, dd) in geo['groups'].items():
             remainder = vk.R_table(sm, sn)(dd) if len(dd) > self.VIA_TABLE_PAIRS else vk.R(sm, sn, dd)
             vals = remainder + vk.homog(sm, sn, dd)
@@ -431,6 +477,21 @@ class Simulation:
             first.add(id(v))
             C[n, n] += zs * (self.st.z[hi] - self.st.z[lo]) / (2 * np.pi * v.a)
         return B, torch.from_numpy(C).to(dev)
+
+    def _via_device(self):
+        """The via blocks' fixed arrays on the device, once: the fans' charge operator and triangles, each
+        interface's triangles with their 6-point quadrature, each span's chains and their axes."""
+        if getattr(self, '_via_dev', None) is None:
+            dev, geo = dv.device(), self._via_geometry()
+            on = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)
+            self._via_dev = dict(
+                dF=on(self.dF, dv.c128), F=on(self.F, torch.int64),
+                tri={k: (on(idx, torch.int64), on(self.P6[idx, :, 0], dv.f64), on(self.P6[idx, :, 1], dv.f64),
+                         on(self.W6[idx], dv.f64)) for k, idx in ((k, self.tri_idx[k]) for k in self.layers)},
+                x={s: on(geo['x'][ns], dv.f64) for s, ns in geo['by_span'].items()},
+                y={s: on(geo['y'][ns], dv.f64) for s, ns in geo['by_span'].items()},
+                ns={s: on(ns, torch.int64) for s, ns in geo['by_span'].items()})
+        return self._via_dev
 
     def _div_ops(self):
         """x (Nt, k) -> D^T x (Ne, k) on the device: each edge's +l / A_+ on T+ and -l / A_- on T- (base.py:47)."""
@@ -474,7 +535,7 @@ class Simulation:
 
     def port_current(self, I, j):
         """Port j's current for each excitation column of I: its chain's current (signed) or the current through
-        its gap line."""
+        its gap line (a wave port's: at its gap)."""
         p = self.ports[j]
         if isinstance(p, ViaPort):
             return p.sign * I[self.Ne + self._port_chain(p)]
@@ -490,20 +551,53 @@ class Simulation:
                                      f"(N = {len(I)}): a degenerate mesh, or a port with no return path")
         return I
 
+    def cut_current(self, I, j):
+        """The current through each sample cut of wave port j (ncut x ncol) (sim.py:1005-1007)."""
+        return np.array([(sig * self.rwg.l[idx]) @ I[idx] for idx, sig in self.cuts[j]])
+
     def sparams(self, f, keep=False):
-        """S at f for the lumped ports (sim.py:1045-1092)."""
+        """S at f (sim.py:1045-1092)."""
         return self.sparams_from(f, self.currents(f, self.excitation()), keep)
 
     def sparams_from(self, f, I, keep=False):
-        """S at f from the basis currents I of the excitation() columns: each port driven by 1 V in turn, its
-        incident and reflected waves a = (V + Zc I) / (2 sqrt Zc), b = (V - Zc I) / (2 sqrt Zc), S = B A^-1."""
-        zc = np.array([p.zc for p in self.ports], float)
-        Ip = np.array([self.port_current(I, j) for j in range(len(self.ports))])    # (port, excitation)
-        V = np.eye(len(self.ports))
-        root = (2 * np.sqrt(zc))[:, None]
-        A, B = (V + zc[:, None] * Ip) / root, (V - zc[:, None] * Ip) / root
+        """S at f from the basis currents I of the excitation() columns (each port driven in turn): a lumped
+        port's incident and reflected waves a = (V + Zc I) / (2 sqrt Zc), b = (V - Zc I) / (2 sqrt Zc) with V the
+        1 V EMF; a wave port's from its feed's forward and backward current waves (a e^(-gamma s), b e^(gamma s),
+        s from its reference plane), sqrt(Zc) a and -sqrt(Zc) b; S = B A^-1, made reciprocal by balance() when it
+        has wave ports. self.last keeps the fit: gamma and its residual per wave port."""
+        for p in self.ports:
+            if getattr(p, 'zc_fn', None) is not None:
+                p.zc = float(p.zc_fn(f))
+        n = len(self.ports)
+        A, B = np.zeros((n, n), complex), np.zeros((n, n), complex)
+        gam, resid = np.full(n, np.nan + 0j), np.zeros(n)
+        V = np.eye(n)
+        for i, p in enumerate(self.ports):
+            rz = np.sqrt(p.zc)
+            if isinstance(p, LinePort):
+                s, Ik = self.samples[i], self.cut_current(I, i)
+                known = p.gamma_fn is not None
+                g = gam[i] = complex(p.gamma_fn(f)) if known else wave_gamma(Ik, abs(s[1] - s[0]))
+                a, b = wave_split(Ik, s - p.ref, g)
+                fit = np.exp(-g * (s - p.ref))[:, None] * a[None, :] + np.exp(g * (s - p.ref))[:, None] * b[None, :]
+                resid[i] = np.linalg.norm(Ik - fit, axis=0).max() / np.linalg.norm(Ik, axis=0).max()
+                A[i], B[i] = rz * a, -rz * b
+                span = abs(g.imag) * abs(s[-1] - s[0])
+                if not known and span < FIT_SPAN:
+                    self.log.note(f"WARNING {f / 1e9:.2f} GHz port {p.name}: its cuts span {span:.2f} rad - the "
+                                  "wave separation is unreliable (a longer straight feed, or a gap or via port)")
+                if resid[i] > 1e-2:
+                    self.log.note(f"WARNING {f / 1e9:.2f} GHz port {p.name}: wave-fit residual {resid[i]:.1e} - "
+                                  "cuts too close to a discontinuity, or too short a feed at this frequency")
+            else:
+                Ip = self.port_current(I, i)
+                A[i], B[i] = (V[i] + p.zc * Ip) / (2 * rz), (V[i] - p.zc * Ip) / (2 * rz)
         S = B @ np.linalg.inv(A)
-        self.last = dict(f=f, I=I if keep else None, A=A, B=B)
+        wave = np.array([isinstance(p, LinePort) for p in self.ports])
+        scale = np.ones(n)
+        if wave.any() and n > 1:
+            S, scale = balance(S, ~wave)
+        self.last = dict(f=f, I=I if keep else None, A=A, B=B, gamma=gam, resid=resid, scale=scale)
         return S
 
     def sweep(self, freqs):

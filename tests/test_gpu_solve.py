@@ -133,6 +133,43 @@ def test_rounded_double_factor_preconditions_as_the_double_one(gpu):
     assert max(abs(x - y) for x, y in zip(runs['c64'][1], runs['c128'][1])) <= 2
 
 
+def _series_line(Z):
+    """The reference's series-resistor board (tests/test_solver.py:339-350): a 0.35 mm strip, 12 mm long, on 0.2 mm
+    of er 4.3, wave ports 0.5 mm in from both ends looking inwards (cuts from 1.5 mm every 0.7 mm to 4 mm), a series
+    element Z at the middle; here the reference planes are at the element (5.5 mm in) instead of 4 mm in, so that
+    circuit theory applies without a line section on either side."""
+    from mom25d_amd.multilayer import Layer, LayerStack
+    from mom25d_amd.sim import LinePort, SeriesLumped
+    w, h, L = 0.35e-3, 0.2e-3, 12e-3
+    st = LayerStack([Layer(h, 4.3)])
+    p1 = LinePort(1, (-L / 2 + 0.5e-3, 0), (1, 0), w, 5.5e-3, s0=1.5e-3, step=0.7e-3, s1=4e-3, name='P1')
+    p2 = LinePort(1, (L / 2 - 0.5e-3, 0), (-1, 0), w, 5.5e-3, s0=1.5e-3, step=0.7e-3, s1=4e-3, name='P2')
+    return Simulation(st, {1: [(-L / 2, -w / 2, L / 2, w / 2)]}, [p1, p2],
+                      lumped=[SeriesLumped(1, (0, 0), (1, 0), w, Z)], dmax=0.25e-3, w_ref=w, verbose=False)
+
+
+def test_wave_ports_see_a_series_element_as_circuit_theory_does(gpu):
+    """Wave ports on the reference's series-resistor line (its own test: |S21| = 2 Z0 / (2 Z0 + 50) within 0.01).
+    The ports' S is in the feed's own impedance Z0 whatever zc both share, so without the reference's line
+    characterisation the circuit identities that hold for any Z0 are checked, with the reference planes at the
+    element: a series element gives S21 = 1 - S11 (50 ohms, within 0.01), the Z0 it implies, 50 (1 - S11) / (2 S11), is the 0.35 mm / 0.2 mm strip's (40-60 ohms,
+    nearly real), and S21 then matches 2 Z0 / (2 Z0 + 50); a lossless element (j 50 ohms) on the lossless line
+    conserves power (|S11|^2 + |S21|^2 within 0.01 of 1); S is symmetric; the fitted gamma's phase constant lies
+    between the free-space and the substrate's wavenumbers and its residual is small."""
+    f = 5e9
+    sim = _series_line(50.0)
+    S = sim.sparams(f)
+    assert abs(S[1, 0] - (1 - S[0, 0])) < 0.01 and abs(S[0, 0] - S[1, 1]) < 0.01 and abs(S[0, 1] - S[1, 0]) < 1e-9
+    z0 = 50.0 * (1 - S[0, 0]) / (2 * S[0, 0])
+    assert 40 < z0.real < 60 and abs(z0.imag) < 0.1 * z0.real
+    assert abs(abs(S[1, 0]) - 2 * z0.real / (2 * z0.real + 50)) < 0.01
+    k0 = 2 * np.pi * f / 299792458.0
+    g = sim.last['gamma']
+    assert np.all((g.imag > k0) & (g.imag < k0 * np.sqrt(4.3))) and sim.last['resid'].max() < 1e-2
+    Sx = _series_line(50j).sparams(f)
+    assert abs(np.sum(np.abs(Sx[:, 0]) ** 2) - 1) < 0.01
+
+
 def test_matrix_free_operator_and_lean_path(gpu):
     """matvec_exact (far field triangle-aggregated, near correction as a sparse matrix, via rows and columns) equals
     the dense FP64 Z times X to 1e-12 of each column's largest value on the coupled-line board (N ~ 3k, 4 via
