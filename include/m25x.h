@@ -256,6 +256,13 @@ int m25x_getrs_noalloc(int dtype, const void *LU, long n, long lda, const int *i
 int m25x_lu_set_gemm_stage(int ks);
 int m25x_lu_gemm_stage(void);
 
+/* Look-ahead of the tournament LU (m25x_getrf, one matrix), 1 on, 0 off (the default): the next panel's tournament
+ * runs on a second stream of the library's (one per device, the default priority) beside the rest of the trailing
+ * update on the caller's stream. The factor is the same bit for bit either way (the same kernels; each update
+ * split by columns). No counterpart in the reference (its LU is LAPACK's or a vendor library's). */
+int m25x_lu_set_lookahead(int on);
+int m25x_lu_lookahead(void);
+
 /* The matrix cores of the complex64 tournament LU's trailing update, for every later call in the process: 1 (the
  * default) FP32 with FP32 accumulation (twice the FP64 peak rate), 0 FP64 (products and sums in FP64, one rounding
  * to FP32 an update: a smaller backward error). The GPU backend's complex64 factor (src/mom25d/gpu.py:69-74) runs

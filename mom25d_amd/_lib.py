@@ -89,6 +89,8 @@ _SIGS = {
     'm25x_lu_set_gemm_stage': ([I], I),
     'm25x_lu_gemm_stage': ([], I),
     'm25x_lu_set_update_precision': ([I], I),
+    'm25x_lu_set_lookahead': ([I], I),
+    'm25x_lu_lookahead': ([], I),
     'm25x_getrf_batched': ([I, I, P, P, P], I),
     'm25x_getrs_batched': ([I, I, P, P, I, P, P], I),
     'm25x_zgemm_nk_batched': ([I, P, P, P], I),

@@ -21,6 +21,7 @@
 // values on as self-marking words.
 #include <atomic>
 #include <mutex>
+#include <vector>
 #include "m25x_lu_dev.h"
 
 using namespace m25x;
@@ -950,10 +951,103 @@ int getrf_t(T *A, long n, long lda, int *ipiv, int *info, void *work, size_t wby
     return M25X_OK;
 }
 
+// ---- look-ahead: the next panel's tournament beside the trailing update -----------------------------------
+// The tournament of panel j + 1 needs only panel j + 1's columns updated, so the update of step j is split: the next
+// panel's 64 columns first, then the rest, and the tournament runs on a second stream beside the rest (its waves
+// raise their own issue priority on the SIMDs they share with the update, s_setprio; both streams keep the default
+// queue priority). Off by default (m25x_lu_set_lookahead): one extra stream per device, created once.
+std::atomic<int> g_lookahead{0};
+
+struct LookAhead {
+    hipStream_t p = nullptr;                  // the panel stream; the update runs on the caller's
+    hipEvent_t ev[4] = {};
+    bool ok = false;
+};
+
+LookAhead *lookahead_streams()
+{
+    static std::mutex mu;
+    static LookAhead la[64];
+    int dev = 0;
+    if (!g_lookahead.load(std::memory_order_relaxed) || hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+        return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    LookAhead &L = la[dev];
+    if (L.ok) return &L;
+    if (hipStreamCreateWithFlags(&L.p, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    for (auto &e : L.ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    L.ok = true;
+    return &L;
+}
+
 template <typename T> size_t tour_work_bytes(long n)
 {
     const long nn = tour_nodes(n, Tour<T>::RL, Tour<T>::AR);
     return nn * sizeof(TNode<T>) + (size_t)TW * TW * sizeof(T) + sizeof(int) * ((size_t)nn + 4 * TW + 64);
+}
+
+// The tournament LU with one panel of look-ahead (see LookAhead): the same kernels and arithmetic as the plain loop
+// (each trailing update split by columns into two launches), so the factor is the same bit for bit. The caller's
+// stream runs the row moves, L21, U12 and the update, stream p the tournaments; events order them.
+template <typename T>
+int getrf_tp_lookahead(T *A, long n, long lda, int *ipiv, int *perm, int *info, TNode<T> *nodes, int *arrive, T *top,
+                       int *pairs, LookAhead &la, hipStream_t s)
+{
+    constexpr int RL = Tour<T>::RL;
+    const int dt = dtype_of<T>();
+    const double es = sizeof(T);
+    hipStream_t g = s, p = la.p;
+    hipEvent_t start = la.ev[0], panel = la.ev[1], next = la.ev[2], done = la.ev[3];
+    if (hipEventRecord(start, s) != hipSuccess || hipStreamWaitEvent(p, start, 0) != hipSuccess)
+        return fail(M25X_EHIP, "getrf: look-ahead stream");
+    auto tournament = [&](long j0) -> int {
+        const int jb = (int)std::min((long)TW, n - j0);
+        const long m = n - j0;
+        TimedLaunch tl(p, M25X_K_TSLU, dt, es * m * jb, 0.0);
+        if (int rc = launch_tslu((const T *)A, lda, n, j0, jb, (int)tour_leaves(m, RL), nodes, arrive, top, ipiv, pairs,
+                                 info, p))
+            return rc;
+        return hipEventRecord(panel, p) == hipSuccess ? M25X_OK : fail(M25X_EHIP, "getrf: event");
+    };
+    auto update = [&](long j0, int jb, long c0, long c1, long c2) {       // A22[c0:, c1:c2] -= L21 U12[:, c1:c2]
+        const long M = n - c0, N = c2 - c1;
+        TimedLaunch tl(g, M25X_K_GEMM, dt, es * (2.0 * M * N + (M + N) * jb), 8.0 * M * N * jb);
+        launch_update<T>(g, dim3(blocks(M, GM), blocks(N, GN)), A + c1 * lda + c0, lda,
+                         (const T *)(A + j0 * lda + c0), lda, (const T *)(A + c1 * lda + j0), lda, M, N, (long)jb);
+    };
+    if (int rc = tournament(0)) return rc;
+    for (long j0 = 0; j0 < n; j0 += TW) {
+        const int jb = (int)std::min((long)TW, n - j0);
+        const long m = n - j0;
+        if (hipStreamWaitEvent(g, panel, 0) != hipSuccess) return fail(M25X_EHIP, "getrf: wait");
+        {
+            TimedLaunch tl(g, M25X_K_APPLY_MOVES, dt, 2.0 * es * 2 * jb * n, 0.0);
+            hipLaunchKernelGGL(k_apply_moves<T>, dim3(blocks(n, 8) + (perm ? 1 : 0)), dim3(256), 0, g, A, lda, 0L, n,
+                               (const int *)pairs, j0, jb, (const T *)top, perm);
+        }
+        if (m > jb) {
+            TimedLaunch tl(g, M25X_K_L21, dt, 2.0 * es * (m - jb) * jb, 4.0 * (m - jb) * jb * jb);
+            hipLaunchKernelGGL(k_l21<T>, dim3(blocks(m - jb, 256)), dim3(256), 0, g, A, lda, j0 + jb, m - jb, j0, jb,
+                               (const T *)top);
+        }
+        const long c0 = j0 + jb;
+        if (c0 < n) {
+            {
+                TimedLaunch tl(g, M25X_K_TRSM, dt, 2.0 * es * (n - c0) * jb, 4.0 * (n - c0) * jb * jb);
+                hipLaunchKernelGGL((k_trsm<T, TW>), dim3(blocks(n - c0, TC)), dim3(256), 0, g, A, lda, n, j0, jb, c0);
+            }
+            const long c1 = std::min(n, c0 + TW);
+            update(j0, jb, c0, c0, c1);                                   // the next panel's columns
+            if (hipEventRecord(next, g) != hipSuccess || hipStreamWaitEvent(p, next, 0) != hipSuccess)
+                return fail(M25X_EHIP, "getrf: event");
+            if (int rc = tournament(c0)) return rc;                       // beside the rest of the update
+            if (c1 < n) update(j0, jb, c0, c1, n);
+        }
+        if (int rc = check_launch("getrf/look-ahead")) return rc;
+    }
+    (void)done;                               // the last panel's tournament was waited for on the caller's stream
+    return M25X_OK;
 }
 
 template <typename T>
@@ -973,6 +1067,9 @@ int getrf_tp(T *A, long n, long lda, int *ipiv, int *perm, int *info, void *work
     if (info) hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, info, 1);
     if (perm) hipLaunchKernelGGL(k_iota, dim3(blocks(n, 256)), dim3(256), 0, s, perm, n);
     if (int rc = check_launch("getrf")) return rc;
+    if (n > 2 * TW)
+        if (LookAhead *la = lookahead_streams())
+            return getrf_tp_lookahead<T>(A, n, lda, ipiv, perm, info, nodes, arrive, top, pairs, *la, s);
     for (long j0 = 0; j0 < n; j0 += TW) {
         const int jb = (int)std::min((long)TW, n - j0);
         const long m = n - j0;
@@ -1491,6 +1588,15 @@ int m25x_getrf(int dtype, int pivot, void *A, long n, long lda, int *ipiv, int *
     }
     return rc;
 }
+
+int m25x_lu_set_lookahead(int on)
+{
+    if (on != 0 && on != 1) return fail(M25X_EINVAL, "lu_set_lookahead: 0 or 1, not %d", on);
+    g_lookahead.store(on, std::memory_order_relaxed);
+    return M25X_OK;
+}
+
+int m25x_lu_lookahead(void) { return g_lookahead.load(std::memory_order_relaxed); }
 
 size_t m25x_getrs_work_bytes(int dtype, long n, long ldb)
 {

@@ -333,6 +333,7 @@ __global__ __launch_bounds__(2 * Tour<T>::RL) void k_tslu(const T *A, long lda, 
                                                          TNode<T> *nodes, int *arrive, T *top, int *ipiv, int *pairs,
                                                          int *info)
 {
+    __builtin_amdgcn_s_setprio(3);    // ahead of the trailing update's waves where the LU's look-ahead runs it beside
     tslu_body<T>(A, lda, n, j0, jb, P, nodes, arrive, top, ipiv, pairs, info, blockIdx.x);
 }
 

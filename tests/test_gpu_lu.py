@@ -151,6 +151,28 @@ def test_getrf_c64_tournament(gpu, n):
 
 
 @pytest.mark.parametrize('dt', [np.complex64, np.complex128])
+def test_lookahead_factor_is_the_same_bit_for_bit(gpu, dt):
+    """The tournament LU with look-ahead (the next panel's tournament on a second stream beside the rest of the
+    update) and without it (the default) give the same factor, pivots and permutation bit for bit, from three
+    panels with a one-column last one to many."""
+    from mom25d_amd import _lib as L
+    dv = gpu
+    was = L.lib().m25x_lu_lookahead()
+    for n in (129, 200, 777, 3001):
+        Z = seeded_matrix(n, dtype=dt)
+        outs = []
+        for la in (1, 0):
+            L.call('m25x_lu_set_lookahead', la)
+            try:
+                F = _factor(dv, Z, 'tournament')
+                outs.append((F.LU.cpu(), F.ipiv.cpu(), F.perm.cpu(), int(F.info.item())))
+            finally:
+                L.call('m25x_lu_set_lookahead', was)
+        (a, pa, qa, ia), (b, pb, qb, ib) = outs
+        assert torch.equal(a, b) and torch.equal(pa, pb) and torch.equal(qa, qb) and ia == ib == 0, n
+
+
+@pytest.mark.parametrize('dt', [np.complex64, np.complex128])
 def test_getrs_right_hand_side_counts(gpu, dt):
     """The solves for 1 to 8 right-hand sides at sizes with and without a partial last 64-row block and with long
     chains of 64-row blocks (one workgroup each, every hand-off between them in play), both transposes, against
