@@ -664,7 +664,8 @@ __device__ __attribute__((always_inline)) void subst(T (&t)[NW], const T (*Ms)[S
 //  - at its start it stages its diagonal block's operator in LDS (strict triangle, rows scaled by 1 / U_ii) and
 //    reads its right-hand sides;
 //  - it then meets the earlier blocks a in order: thread (row lane, columns 8 wv + u of block a) holds its
-//    entries of Op_qa, loaded TrsvDepth blocks ahead, and adds Op_qa y_a to its partial sums once y_a is out;
+//    entries of Op_qa, loaded TrsvDepth blocks ahead, and subtracts Op_qa y_a from its partial sums once y_a is
+//    out (fused multiply-subtracts: four per complex term);
 //  - after the last one (a = q - 1) the partial sums meet in LDS, waves 0-3 form the block's right-hand side
 //    (wave w takes right-hand sides w and, for NR = 8, w + 4; lane i row i), substitute, and publish y_q eight
 //    rows at a time as the substitution finishes them: the next workgroup's wave that takes those rows starts its
@@ -753,7 +754,7 @@ __device__ __forceinline__ void trsv_body(const T *LU, long lda, long n, const T
     if (D > 2 && q > 2) load_tile(m2, order(2));
     if constexpr (sizeof(T) != 16) stage();
 
-    T s[NR];
+    T s[NR];                                 // minus the wave's share of the products
 #pragma unroll
     for (int r = 0; r < NR; ++r) s[r] = zero;
     // lane l < 8 NR of wave wv takes y of row 8 wv + l / NR of block a, right-hand side l % NR (zero for the
@@ -795,7 +796,7 @@ __device__ __forceinline__ void trsv_body(const T *LU, long lda, long n, const T
 #pragma unroll
         for (int u = 0; u < TE; ++u) {
 #pragma unroll
-            for (int r = 0; r < NR; ++r) s[r] = tadd(s[r], tmul(mt[u], ybw[u * NR + r]));
+            for (int r = 0; r < NR; ++r) s[r] = cfms(s[r], mt[u], ybw[u * NR + r]);
             if constexpr (sizeof(T) == 16) __builtin_amdgcn_sched_barrier(0);     // y read as it is used
         }
         __builtin_amdgcn_wave_barrier();
@@ -822,7 +823,7 @@ __device__ __forceinline__ void trsv_body(const T *LU, long lda, long n, const T
 #pragma unroll
             for (int sl = 0; sl + st < NWV; sl += 2 * st) h[sl] = tadd(h[sl], h[sl + st]);
         const T v = h[0];
-        t[w] = csel(inrow && rr[w] < nrhs, tmul(tsub(bxs[rr[w]][lane], v), rds[lane]), zero);
+        t[w] = csel(inrow && rr[w] < nrhs, tmul(tadd(bxs[rr[w]][lane], v), rds[lane]), zero);
     }
     auto pub = [&](int lo) __attribute__((always_inline)) {
         if (lane >= lo && lane < lo + 8 && inrow) {
