@@ -8,40 +8,31 @@ namespace {
 
 // The pivot search orders candidates by (magnitude, then the smaller key): magnitudes |Re| + |Im| are non-negative,
 // so their IEEE bit patterns order as unsigned integers (a dead row has bits 0 and the largest key, so any live row,
-// even one with a zero entry, outranks it). Wave reductions in two passes - the largest bits, then the smallest
-// key among the lanes holding them - each within rows of 16 lanes by DPP exchanges (quad xor 1, quad xor 2, half-row
-// mirror, row mirror), then over the four rows by readlane.
+// even one with a zero entry, outranks it). Wave reductions in 32-bit passes - the largest magnitude word(s), then
+// the smallest key among the lanes holding them - each within rows of 16 lanes by DPP exchanges (quad xor 1, quad
+// xor 2, half-row mirror, row mirror), then over the four rows by readlane.
 typedef unsigned long long u64;
 
-template <int CTRL> __device__ inline u64 dpp_u64(u64 v)
+// 32-bit wave reductions: four DPP rounds within rows of 16 lanes (each one instruction: the exchange is the
+// operation's source modifier, the old value the operation's identity), then the four rows by readlane
+__device__ inline unsigned wave_max_u32(unsigned v)
 {
-    const unsigned lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)v, CTRL, 0xF, 0xF, false);
-    const unsigned hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), CTRL, 0xF, 0xF, false);
-    return ((u64)hi << 32) | lo;
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));
+    const unsigned a = max((unsigned)__builtin_amdgcn_readlane((int)v, 0), (unsigned)__builtin_amdgcn_readlane((int)v, 16));
+    const unsigned b = max((unsigned)__builtin_amdgcn_readlane((int)v, 32), (unsigned)__builtin_amdgcn_readlane((int)v, 48));
+    return max(a, b);
 }
-__device__ inline u64 readlane_u64(u64 v, int l)
+__device__ inline int wave_min_i32(int v)
 {
-    const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)v, l), hi = __builtin_amdgcn_readlane((int)(unsigned)(v >> 32), l);
-    return ((u64)hi << 32) | lo;
-}
-
-__device__ inline void wave_best(u64 &vb, int &key)
-{
-    u64 m = vb;
-    m = max(m, dpp_u64<0xB1>(m));      // quad_perm [1, 0, 3, 2]
-    m = max(m, dpp_u64<0x4E>(m));      // quad_perm [2, 3, 0, 1]
-    m = max(m, dpp_u64<0x141>(m));     // row_half_mirror
-    m = max(m, dpp_u64<0x140>(m));     // row_mirror
-    m = max(max(readlane_u64(m, 0), readlane_u64(m, 16)), max(readlane_u64(m, 32), readlane_u64(m, 48)));
-    int k = vb == m ? key : 0x7fffffff;
-    k = min(k, __builtin_amdgcn_update_dpp(0, k, 0xB1, 0xF, 0xF, false));
-    k = min(k, __builtin_amdgcn_update_dpp(0, k, 0x4E, 0xF, 0xF, false));
-    k = min(k, __builtin_amdgcn_update_dpp(0, k, 0x141, 0xF, 0xF, false));
-    k = min(k, __builtin_amdgcn_update_dpp(0, k, 0x140, 0xF, 0xF, false));
-    k = min(min(__builtin_amdgcn_readlane(k, 0), __builtin_amdgcn_readlane(k, 16)),
-            min(__builtin_amdgcn_readlane(k, 32), __builtin_amdgcn_readlane(k, 48)));
-    vb = m;
-    key = k;
+    v = min(v, __builtin_amdgcn_update_dpp(0x7fffffff, v, 0xB1, 0xF, 0xF, false));
+    v = min(v, __builtin_amdgcn_update_dpp(0x7fffffff, v, 0x4E, 0xF, 0xF, false));
+    v = min(v, __builtin_amdgcn_update_dpp(0x7fffffff, v, 0x141, 0xF, 0xF, false));
+    v = min(v, __builtin_amdgcn_update_dpp(0x7fffffff, v, 0x140, 0xF, 0xF, false));
+    return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
 }
 
 __device__ inline u64 mag_bits(float2 a) { return (u64)__float_as_uint(fabsf(a.x) + fabsf(a.y)); }
@@ -101,15 +92,13 @@ __device__ __attribute__((always_inline)) int reg_gepp(RowH<T> &x, int nr, int n
         int key = cand ? ((pos << 16) | row) : 0x7fffffff;
         int gk;
         if constexpr (sizeof(T) == 8) {
-            // complex64: the magnitude's 32 bits above the complemented key, so one maximum picks the largest
-            // magnitude and, among equals, the smallest key (a dead row: 0, below every live one)
-            u64 pk = cand ? (vb << 32) | (u64)(0xffffffffu - (unsigned)key) : 0ull;
-            pk = max(pk, dpp_u64<0xB1>(pk));
-            pk = max(pk, dpp_u64<0x4E>(pk));
-            pk = max(pk, dpp_u64<0x141>(pk));
-            pk = max(pk, dpp_u64<0x140>(pk));
-            pk = max(max(readlane_u64(pk, 0), readlane_u64(pk, 16)), max(readlane_u64(pk, 32), readlane_u64(pk, 48)));
-            key = pk ? (int)(0xffffffffu - (unsigned)pk) : 0x7fffffff;
+            // complex64: the wave's largest magnitude (32 bits; a dead row 0), then the smallest key among the
+            // lanes holding it (a dead row's key is the largest, so a live row with a zero entry wins over it). The
+            // wave's slot packs the pair as the magnitude above the complemented key, so that one maximum over the
+            // slots picks the largest magnitude and, among equals, the smallest key (0: no live row in the wave).
+            const unsigned mx = wave_max_u32((unsigned)vb);
+            key = wave_min_i32((unsigned)vb == mx ? key : 0x7fffffff);
+            const u64 pk = key != 0x7fffffff ? ((u64)mx << 32) | (u64)(0xffffffffu - (unsigned)key) : 0ull;
             if ((key & 0xffff) == row && key != 0x7fffffff) {     // this wave's best row publishes itself
                 auto pub = [&](auto jc) __attribute__((always_inline)) {
                     constexpr int j = decltype(jc)::value;
@@ -124,7 +113,13 @@ __device__ __attribute__((always_inline)) int reg_gepp(RowH<T> &x, int nr, int n
             for (int q = 1; q < NW; ++q) g = max(g, sh.vb[b][q]);
             gk = (int)(0xffffffffu - (unsigned)g);
         } else {
-            wave_best(vb, key);
+            // complex128: the largest 64-bit magnitude as (high word, then low word among the lanes holding the
+            // largest high word), then the smallest key among the lanes holding both
+            const unsigned hi = (unsigned)(vb >> 32), lo = (unsigned)vb;
+            const unsigned mh = wave_max_u32(hi);
+            const unsigned ml = wave_max_u32(hi == mh ? lo : 0u);
+            key = wave_min_i32(hi == mh && lo == ml ? key : 0x7fffffff);
+            vb = ((u64)mh << 32) | ml;
             if ((key & 0xffff) == row && key != 0x7fffffff) {     // this wave's best row publishes itself
                 auto pub = [&](auto jc) __attribute__((always_inline)) {
                     constexpr int j = decltype(jc)::value;
@@ -152,7 +147,8 @@ __device__ __attribute__((always_inline)) int reg_gepp(RowH<T> &x, int nr, int n
             const T a = csel(h == hk, mine, other);
             if (pos == k) pos = p;
             // the multiplier by the pivot's reciprocal conj(pv) / |pv|^2, as LAPACK's getf2 scales by 1 / pivot (a
-            // zero pivot leaves the column as it is)
+            // zero pivot leaves the column as it is); formed here, where it overlaps the update's LDS reads (formed
+            // by the pivot row's wave before the barrier it measured slower)
             const T pv = prow[k];
             typedef typename CX<T>::R R;
             const R d = pv.x * pv.x + pv.y * pv.y;
