@@ -298,6 +298,7 @@ __device__ __forceinline__ void gemm_body(T *C, long ldc, const T *A, long lda, 
     typedef typename CX<AC>::R R;           // the matrix cores' type: products and sums in it, one rounding to T
     typedef typename CX<T>::R TR;
     typedef typename CX<AC>::acc4 acc4;
+    constexpr int LA = GM * KS / 256, LB = KS * GN / 256;   // a thread's elements of A and of B a stage
     __shared__ T As[KS][GM];         // As[p][i] = A[i][p]
     __shared__ T Bs[KS][GN + 1];     // Bs[p][j] = B[p][j]
     const long m0 = bx * GM, n0 = by * GN;
@@ -311,19 +312,37 @@ __device__ __forceinline__ void gemm_body(T *C, long ldc, const T *A, long lda, 
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) { cr[a][b][r] = 0; ci[a][b][r] = 0; }
-    for (long k0 = 0; k0 < K; k0 += KS) {
-        // stage A (GM x KS) and B (KS x GN); zero outside
-        for (int idx = tid; idx < GM * KS; idx += 256) {
-            const int i = idx % GM, p = idx / GM;
+    // a stage's A (GM x KS) and B (KS x GN) elements of the thread, zero outside, read into registers one stage
+    // ahead: the loads of stage s + 1 are in flight while stage s is multiplied
+    T ra[LA], rb[LB];
+    auto fetch = [&](long k0) __attribute__((always_inline)) {
+#pragma unroll
+        for (int it = 0; it < LA; ++it) {
+            const int idx = tid + it * 256, i = idx % GM, p = idx / GM;
             const long gi = m0 + i, gp = k0 + p;
-            As[p][i] = ldz(gi < M && gp < K, A + gp * lda + gi);
+            ra[it] = ldz(gi < M && gp < K, A + gp * lda + gi);
         }
-        for (int idx = tid; idx < KS * GN; idx += 256) {
-            const int p = idx % KS, j = idx / KS;
+#pragma unroll
+        for (int it = 0; it < LB; ++it) {
+            const int idx = tid + it * 256, p = idx % KS, j = idx / KS;
             const long gp = k0 + p, gj = n0 + j;
-            Bs[p][j] = ldz(gp < K && gj < N, B + gj * ldb + gp);
+            rb[it] = ldz(gp < K && gj < N, B + gj * ldb + gp);
+        }
+    };
+    fetch(0);
+    for (long k0 = 0; k0 < K; k0 += KS) {
+#pragma unroll
+        for (int it = 0; it < LA; ++it) {
+            const int idx = tid + it * 256;
+            As[idx / GM][idx % GM] = ra[it];
+        }
+#pragma unroll
+        for (int it = 0; it < LB; ++it) {
+            const int idx = tid + it * 256;
+            Bs[idx % KS][idx / KS] = rb[it];
         }
         __syncthreads();
+        if (k0 + KS < K) fetch(k0 + KS);
 #pragma unroll
         for (int ks = 0; ks < KS / 2; ++ks) {
             const int p = 2 * ks + ph;
@@ -349,7 +368,20 @@ __device__ __forceinline__ void gemm_body(T *C, long ldc, const T *A, long lda, 
         }
         __syncthreads();
     }
-    // lane (li, kk), register r: C row m0 + wm + a*16 + li, column n0 + wn + b*16 + out_row(kk, r)
+    // lane (li, kk), register r: C row m0 + wm + a*16 + li, column n0 + wn + b*16 + out_row(kk, r); for C -= A B
+    // the thread's entries of C are all read first (in flight together), then written
+    T cv[2][2][4];
+    if (SUB) {
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const long gi = m0 + wm + a * 16 + li, gj = n0 + wn + b * 16 + CX<AC>::out_row(kk, r);
+                    cv[b][a][r] = ldz(gi < M && gj < N, C + gj * ldc + gi);
+                }
+    }
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
@@ -360,7 +392,7 @@ __device__ __forceinline__ void gemm_body(T *C, long ldc, const T *A, long lda, 
                 if (gi < M && gj < N) {
                     T *c = C + gj * ldc + gi;
                     if (SUB) {
-                        const T v = *c;
+                        const T v = cv[b][a][r];
                         *c = CX<T>::make((TR)((R)v.x - cr[b][a][r]), (TR)((R)v.y - ci[b][a][r]));
                     } else {
                         *c = CX<T>::make((TR)cr[b][a][r], (TR)ci[b][a][r]);
