@@ -249,16 +249,36 @@ __device__ __forceinline__ void trsm_body(T *A, long lda, long n, long j0, int j
     const long cb = c0 + bx * TC;
     const int nc = (int)std::min((long)TC, n - cb);
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < jb * TC; idx += 256) {
-        const int c = idx / jb, r = idx % jb;
-        if (c < nc) X[r][c] = A[(cb + c) * lda + j0 + r];
-    }
     const T *L = A + j0 * lda + j0;          // L11[i][k] = L[k * lda + i]
-    if constexpr (STAGE)
-        for (int idx = tid; idx < jb * jb; idx += 256) {
-            const int k = idx / jb, i = idx % jb;
-            Lt[k][i] = L[(long)k * lda + i];
+    if constexpr (STAGE) {                   // jb <= TW: the loads of either block in flight before the stores
+        constexpr int NX = TW * TC / 256, NL = TW * TW / 256;
+        T vx[NX], vl[NL];
+#pragma unroll
+        for (int u = 0; u < NX; ++u) {
+            const int idx = tid + u * 256, c = idx / jb, r = idx % jb;
+            vx[u] = ldz(idx < jb * TC && c < nc, A + (cb + c) * lda + j0 + r);
         }
+#pragma unroll
+        for (int u = 0; u < NL; ++u) {
+            const int idx = tid + u * 256, k = idx / jb, i = idx % jb;
+            vl[u] = ldz(idx < jb * jb, L + (long)k * lda + i);
+        }
+#pragma unroll
+        for (int u = 0; u < NX; ++u) {
+            const int idx = tid + u * 256, c = idx / jb, r = idx % jb;
+            if (idx < jb * TC && c < nc) X[r][c] = vx[u];
+        }
+#pragma unroll
+        for (int u = 0; u < NL; ++u) {
+            const int idx = tid + u * 256, k = idx / jb, i = idx % jb;
+            if (idx < jb * jb) Lt[k][i] = vl[u];
+        }
+    } else {
+        for (int idx = tid; idx < jb * TC; idx += 256) {
+            const int c = idx / jb, r = idx % jb;
+            if (c < nc) X[r][c] = A[(cb + c) * lda + j0 + r];
+        }
+    }
     __syncthreads();
     for (int k = 0; k < jb - 1; ++k) {
         const int nr = jb - k - 1;
@@ -485,9 +505,15 @@ __device__ __forceinline__ void l21_body(T *A, long lda, long r0, long m, long j
     __shared__ T Us[TW][TW + 1];
     __shared__ T rd[TW];
     const int tid = threadIdx.x;
-    for (int e = tid; e < TW * TW; e += 256) {
-        const int c = e / TW, i = e % TW;
-        Us[i][c] = top[e];
+    {
+        T v[TW * TW / 256];                     // every load in flight before the first store
+#pragma unroll
+        for (int u = 0; u < TW * TW / 256; ++u) v[u] = top[tid + u * 256];
+#pragma unroll
+        for (int u = 0; u < TW * TW / 256; ++u) {
+            const int e = tid + u * 256, c = e / TW, i = e % TW;
+            Us[i][c] = v[u];
+        }
     }
     __syncthreads();
     if (tid < TW) rd[tid] = tid < jb ? trecip(Us[tid][tid]) : CX<T>::make(1, 0);
@@ -504,13 +530,20 @@ __device__ __forceinline__ void l21_body(T *A, long lda, long r0, long m, long j
         };
         Unroll<0, TH>::run(ld);
         if (h == 1) {                           // the first half's solution, final, from the previous pass
-            for (int k = 0; k < 32 && k < jb; ++k) {
-                const T xk = row[(long)k * lda];
-                auto up = [&](auto jc) __attribute__((always_inline)) {
-                    constexpr int j = decltype(jc)::value;
-                    x.set(j, tsub(x.get(j), tmul(xk, Us[k][32 + j])));
-                };
-                Unroll<0, TH>::run(up);
+            for (int k0 = 0; k0 < 32 && k0 < jb; k0 += 8) {
+                T xk[8];                        // eight of its values read together
+#pragma unroll
+                for (int u = 0; u < 8; ++u) xk[u] = ldz(k0 + u < jb, row + (long)(k0 + u) * lda);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int k = k0 + u;
+                    if (k >= jb) break;
+                    auto up = [&](auto jc) __attribute__((always_inline)) {
+                        constexpr int j = decltype(jc)::value;
+                        x.set(j, tsub(x.get(j), tmul(xk[u], Us[k][32 + j])));
+                    };
+                    Unroll<0, TH>::run(up);
+                }
             }
         }
         auto solve = [&](auto jc) __attribute__((always_inline)) {
@@ -559,17 +592,24 @@ __device__ __forceinline__ void apply_moves_body(T *A, long lda, long c0, long n
         if (h == 0 && t < np) perm[dst] = v;
         return;
     }
-    for (long cb = bx * 8; cb < std::min(nc, bx * 8 + 8); cb += 2) {
-        const long c = c0 + cb + h;
-        const bool on = cb + h < nc;
-        T *col = A + c * lda;
-        T v = CX<T>::make(0, 0);
-        if (on && t < np) v = col[src];
-        __syncthreads();
-        if (on && t < np) col[dst] = v;
-        __syncthreads();
-        if (on && c >= j0 && c < j0 + jb)
-            for (int i = t; i < jb; i += 128) col[j0 + i] = top[(c - j0) * TW + i];
+    T v[4];                                         // the workgroup's 8 columns, two at a time: all read first
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const long cb = bx * 8 + 2 * u;
+        v[u] = ldz(cb + h < nc && t < np, A + (c0 + cb + h) * lda + src);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const long cb = bx * 8 + 2 * u, c = c0 + cb + h;
+        if (cb + h < nc && t < np) A[c * lda + dst] = v[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const long cb = bx * 8 + 2 * u, c = c0 + cb + h;
+        if (cb + h < nc && c >= j0 && c < j0 + jb)
+            for (int i = t; i < jb; i += 128) A[c * lda + j0 + i] = top[(c - j0) * TW + i];
     }
 }
 
