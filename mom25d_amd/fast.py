@@ -25,7 +25,11 @@ HYBRID_RETRY = 1.25
 MIXED_RETRY = HYBRID_RETRY
 MIXED_MAX_ITER = 30
 POLISH = 10.0
-FRESH_STALL = (4, 0.1)
+# a fresh factor's refinement stops (the point goes to the complex128 path) when its residual is above 0.1 |b| after
+# 4 iterations, or when from 8 iterations on its rate over the last four would not reach the acceptance level
+# (100 x the tolerance) within the MIXED_MAX_ITER iterations it may take: such a factor does not converge, and
+# iterating on to the limit only delays the complex128 factor that replaces it
+FRESH_STALL = (4, 0.1, MIXED_MAX_ITER, 100.0)
 REUSE_ITER = (8, 60)
 REUSE_KEEP = 4
 # A complex128 fall-back factor is kept for the points after it rounded to complex64 ('c64', the default) or as

@@ -38,7 +38,10 @@ def gmres_batched(A, B, M=None, x0=None, rtol=1e-6, restart=30, maxiter=10, stal
     track: called with the active columns (indices into B's columns, ascending) before A and M see a new set of
     them, for operators that act on each column differently (a batch of systems side by side). stall_each: a
     column that stalls stops alone (the others go on), as for independent systems; otherwise the first stall stops
-    every column, as the reference's lockstep ports do."""
+    every column, as the reference's lockstep ports do. stall (j0, ratio): a column whose residual is still above
+    ratio x |b| after j0 iterations has stalled; (j0, ratio, horizon, level): also one that, from 2 j0 iterations
+    on, would need more than horizon iterations in all to reach level x rtol at its rate over the last four
+    (a column whose residual no longer falls needs infinitely many)."""
     B = B.to(torch.complex128)
     N, k = B.shape
     dev = B.device
@@ -127,6 +130,12 @@ def gmres_batched(A, B, M=None, x0=None, rtol=1e-6, restart=30, maxiter=10, stal
                 break
             if stall is not None and j + 1 >= stall[0]:
                 bad = live & (np.abs(g[j + 1]) > stall[1] * bn[act])
+                if len(stall) > 2 and j + 1 >= 2 * stall[0]:
+                    rate = (rn[j + 1] / np.maximum(rn[j - 3], 1e-300)) ** 0.25
+                    with np.errstate(divide='ignore', invalid='ignore'):
+                        more = np.where(rate < 1, np.log(stall[3] * rtol * bn[act] / np.maximum(rn[j + 1], 1e-300)) /
+                                        np.log(np.minimum(rate, 1 - 1e-12)), np.inf)
+                    bad |= live & (j + 1 + more > stall[2])
                 if stall_each:
                     last[bad] = j
                     live &= ~bad

@@ -80,3 +80,21 @@ def test_device_gmres_matches_oracle(gpu, case):
         assert np.abs(Xd - Xr).max() <= 100 * kw['rtol'] * np.abs(Xr).max()
     else:
         assert itd == itr
+
+
+def test_gmres_rate_horizon(gpu):
+    """stall (j0, ratio, horizon, level): the unpreconditioned seeded system falls at ~0.71 an iteration, so from
+    2 j0 = 8 iterations on it would need ~50 in all to reach 1e-8, beyond a horizon of 30: it stops at the first
+    check instead of running to 30 (the ratio test alone, at 1.0 |b|, never fires). A converging run (the good
+    preconditioner) is the same with and without the horizon, bit for bit."""
+    from mom25d_amd.linalg import gmres_batched
+    A, B, good, _ = _seeded_system()
+    dev = gpu.device()
+    Ad, Bd, Gd = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev), torch.from_numpy(good).to(dev)
+    _, it_full = gmres_batched(lambda V: Ad @ V, Bd, rtol=1e-8, restart=30, maxiter=1, stall=(4, 1.0))
+    _, it_h = gmres_batched(lambda V: Ad @ V, Bd, rtol=1e-8, restart=30, maxiter=1, stall=(4, 1.0, 30, 1.0))
+    assert it_full == [30, 30] and all(8 <= i <= 10 for i in it_h), (it_full, it_h)
+    X0, i0 = gmres_batched(lambda V: Ad @ V, Bd, M=lambda R: Gd @ R, rtol=1e-8, restart=30, maxiter=1, stall=(4, 0.1))
+    X1, i1 = gmres_batched(lambda V: Ad @ V, Bd, M=lambda R: Gd @ R, rtol=1e-8, restart=30, maxiter=1,
+                           stall=(4, 0.1, 30, 100.0))
+    assert i0 == i1 and torch.equal(torch.view_as_real(X0), torch.view_as_real(X1))
