@@ -8,7 +8,9 @@ columns come to the host once per iteration for the conjugated Givens rotations,
 stall cut-off, as the reference computes them. The next iteration's device work (preconditioner, product,
 orthogonalisation) does not depend on those, so it is queued before the host waits for the current column
 (through pinned buffers and an event), unless the residual's recent rate predicts convergence at this iteration;
-an iteration queued past the end is discarded.
+an iteration queued past the end is discarded. Beyond 8 systems (a batch of points side by side) the columns are
+independent systems, and once half of them are done they leave the block in the middle of a cycle, so that a
+few slow ones do not carry the finished ones through every later product and solve.
 """
 import numpy as np
 import torch
@@ -145,6 +147,22 @@ def gmres_batched(A, B, M=None, x0=None, rtol=1e-6, restart=30, maxiter=10, stal
                 elif bad.any():
                     last[live], stalled = j, True
                     break
+            if not fused and live.sum() <= ka // 2:
+                # half the block's columns are done: they leave it (their updates now), and the rest go on alone;
+                # each column's iterations are its own, so theirs do not change, but the products, solves and
+                # orthogonalisation of the columns that are done stop
+                for c in np.flatnonzero(~live):
+                    m = last[c] + 1
+                    if m:
+                        y = np.linalg.solve(np.triu(H[:m, :m, c]), g[:m, c])
+                        X[:, act[c]] += Zs[:m, :, c].T @ torch.from_numpy(y).to(dev)
+                kn = np.flatnonzero(live)
+                kt = torch.from_numpy(kn).to(dev)
+                V, Zs = V[:, :, kt].contiguous(), Zs[:, :, kt].contiguous()
+                H, g, cs, sn, rn = H[:, :, kn], g[:, kn], cs[:, kn], sn[:, kn], rn[:, kn]
+                last, live, act, ka = last[kn], live[kn], act[kn], len(kn)
+                if track is not None:
+                    track(act)
         for c in range(ka):
             m = last[c] + 1
             if not m:
