@@ -40,7 +40,7 @@ def build_sim(precision):
 def sweep(sim, freqs, steps):
     import torch
     fo = sim.fast
-    fo._kept, fo._reach, fo._t_iter, fo._mixed_failed = [], np.log(2.0), None, 0.0
+    fo._kept, fo._reach, fo._t_iter, fo._mixed_failed, fo._last_x = [], np.log(2.0), None, 0.0, None
     lat, S = [], []
     torch.cuda.synchronize()
     t0 = time.perf_counter()

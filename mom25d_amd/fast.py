@@ -48,6 +48,7 @@ class FastOperator:
         self.mixed_tol = 1e-6
         self._mixed_failed = 0.0
         self._kept = []                       # [(f, Factor, dsc (device), seconds a fresh one took)]
+        self._last_x = None                   # the currents (device) of the last point solved
         self._t_iter = None
         self._reach = np.log(2.0)
         self.iters = []
@@ -153,12 +154,13 @@ class FastOperator:
         return Y[:, 0] if one else Y
 
     # ---- solve ---------------------------------------------------------------------------------------------
-    def _refine(self, A, B, M, restart, maxiter, stall=None):
+    def _refine(self, A, B, M, restart, maxiter, stall=None, x0=None):
         """GMRES to mixed_tol: (Y, converged, largest relative residual) (fast.py:658-663). Whether the answer is
         accepted (a residual within 100 x mixed_tol) decides the path as in the reference; an accepted answer whose
         residual is still above POLISH x mixed_tol gets one more restart cycle from itself (S is then within ~3e-5 of
-        the double-precision one rather than ~3e-4)."""
-        Y, self.iters = gmres_batched(A, B, M=M, rtol=self.mixed_tol, restart=restart, maxiter=maxiter, stall=stall)
+        the double-precision one rather than ~3e-4). x0: a guess (a column it does not help starts from zero)."""
+        Y, self.iters = gmres_batched(A, B, M=M, x0=x0, rtol=self.mixed_tol, restart=restart, maxiter=maxiter,
+                                      stall=stall, guess=True)
         rel = self._residual(A, B, Y)
         ok = bool(np.isfinite(rel).all() and (rel <= 100 * self.mixed_tol).all())
         if ok and len(rel) and rel.max() > POLISH * self.mixed_tol:
@@ -166,6 +168,20 @@ class FastOperator:
             self.iters = [a + b for a, b in zip(self.iters, more)]
             rel = self._residual(A, B, Y)
         return Y, ok, float(np.max(rel)) if len(rel) else 0.0
+
+    def _guess(self, dsc, B):
+        """GMRES's first guess at a reused factor: the last solved point's currents in this point's scaling
+        (D^-1 X); along a sweep they are closer to this point's currents than zero is (GMRES still starts a column
+        from zero where the guess is worse). Extrapolating the last two points linearly in frequency, or taking
+        the better of the two guesses column by column, measured no better on the benchmark sweep."""
+        x = self._last_x
+        if x is None or x.shape != B.shape:
+            return None
+        return x / dsc[:, None]
+
+    def _remember(self, X):
+        self._last_x = X
+        return X
 
     @staticmethod
     def _residual(A, B, Y):
@@ -266,13 +282,14 @@ class FastOperator:
             cap = int(np.clip(0.8 * tf0 / max(self._t_iter or 1e-3, 1e-6), *REUSE_ITER))
             with sim.log.timer('refinement (GMRES, reused factorisation)', f"the one of {f0 / 1e9:.4g} GHz, at most {cap}"):
                 t = time.time()
-                Y, ok, rel = self._refine(A, B, lambda R: r * dv.getrs(F0, r * R, trans=1), cap, 1)
+                Y, ok, rel = self._refine(A, B, lambda R: r * dv.getrs(F0, r * R, trans=1), cap, 1,
+                                          x0=self._guess(dsc, B))
                 self._t_iter = (time.time() - t) / max(1, max(self.iters, default=1))
             self._reach_learn(abs(np.log(f / f0)), ok, self.iters, cap)
             if ok:
                 sim.log.note(f"reused LU ({f0 / 1e9:.4g} GHz): GMRES iterations per port {self.iters}")
                 del Z64
-                return dsc[:, None] * Y
+                return self._remember(dsc[:, None] * Y)
             sim.log.note(f"the LU of {f0 / 1e9:.4g} GHz did not precondition {f / 1e9:.4g} GHz in {cap} iterations "
                          f"(residual {rel:.1e}) - a new one")
         if fresh == 'double':
@@ -290,7 +307,7 @@ class FastOperator:
                 self._kept.append((f, F, dsc, time.time() - t_fresh))
                 while len(self._kept) > REUSE_KEEP:
                     self._drop_kept()
-            return dsc[:, None] * Y
+            return self._remember(dsc[:, None] * Y)
         del F, Z64
         sim.log.note(f"WARNING mixed precision did not converge at {f / 1e9:.4g} GHz (residual {rel:.1e}) - solving "
                      f"in double precision up to {MIXED_RETRY * f / 1e9:.4g} GHz")
@@ -306,7 +323,7 @@ class FastOperator:
         with sim.log.timer('LU factor', f"N={Z.shape[0]}, complex128, GPU"):
             Fd = dv.getrf(Z)                                     # in place: the factor owns Z's storage from here
         self._check_info(Fd)
-        X = dsc[:, None] * dv.getrs(Fd, B, trans=1)
+        X = self._remember(dsc[:, None] * dv.getrs(Fd, B, trans=1))
         if sim.reuse != 'off':
             if KEEP_DOUBLE == 'c64':
                 Fd = dv.Factor(Fd.LU.to(dv.c64), Fd.ipiv, Fd.perm, Fd.info)

@@ -35,7 +35,7 @@ def _cnorm(X):
 
 
 def gmres_batched(A, B, M=None, x0=None, rtol=1e-6, restart=30, maxiter=10, stall=None, track=None,
-                  stall_each=False):
+                  stall_each=False, guess=False):
     """(X, iterations per column) for A X = B; A, M: callables on device (N x k) complex128 tensors; B device.
     track: called with the active columns (indices into B's columns, ascending) before A and M see a new set of
     them, for operators that act on each column differently (a batch of systems side by side). stall_each: a
@@ -43,7 +43,8 @@ def gmres_batched(A, B, M=None, x0=None, rtol=1e-6, restart=30, maxiter=10, stal
     every column, as the reference's lockstep ports do. stall (j0, ratio): a column whose residual is still above
     ratio x |b| after j0 iterations has stalled; (j0, ratio, horizon, level): also one that, from 2 j0 iterations
     on, would need more than horizon iterations in all to reach level x rtol at its rate over the last four
-    (a column whose residual no longer falls needs infinitely many)."""
+    (a column whose residual no longer falls needs infinitely many). guess: x0 is only a guess (the solution at
+    a nearby frequency, say): a column whose guess leaves a residual above |b| starts from zero instead."""
     B = B.to(torch.complex128)
     N, k = B.shape
     dev = B.device
@@ -59,6 +60,14 @@ def gmres_batched(A, B, M=None, x0=None, rtol=1e-6, restart=30, maxiter=10, stal
         Xa = X[:, torch.from_numpy(act).to(dev)]
         R = B[:, torch.from_numpy(act).to(dev)] - (A(Xa) if started else 0)
         beta = _cnorm(R)
+        if guess and x0 is not None:
+            guess = False
+            worse = np.flatnonzero(beta > bn[act])
+            if len(worse):                                           # those columns start from zero
+                wc = torch.from_numpy(act[worse]).to(dev)
+                X[:, wc] = 0
+                R[:, torch.from_numpy(worse).to(dev)] = B[:, wc]
+                beta[worse] = bn[act[worse]]
         done = beta <= rtol * bn[act]
         keep = ~done
         act, beta = act[keep], beta[keep]
