@@ -62,7 +62,7 @@ def gmres_batched(A, B, M=None, x0=None, rtol=1e-6, restart=30, maxiter=10, stal
         beta = _cnorm(R)
         if guess and x0 is not None:
             guess = False
-            worse = np.flatnonzero(beta > bn[act])
+            worse = np.flatnonzero(~(beta <= bn[act]))                # (a guess that is not finite, too)
             if len(worse):                                           # those columns start from zero
                 wc = torch.from_numpy(act[worse]).to(dev)
                 X[:, wc] = 0

@@ -101,8 +101,9 @@ def test_gmres_rate_horizon(gpu):
 
 
 def test_gmres_guess(gpu):
-    """guess=True: a column whose guess leaves a residual above |b| starts from zero (the same iterations and the
-    same answer bit for bit as without a guess), a column with a good guess converges in fewer iterations."""
+    """guess=True: a column whose guess leaves a residual above |b|, or not finite, starts from zero (the same
+    iterations and the same answer bit for bit as without a guess), a column with a good guess converges in fewer
+    iterations."""
     from mom25d_amd.linalg import gmres_batched
     A, B, good, _ = _seeded_system()
     dev = gpu.device()
@@ -118,3 +119,6 @@ def test_gmres_guess(gpu):
     assert i1[1] < i0[1], (i0, i1)
     res = np.linalg.norm(A @ X1.cpu().numpy() - B, axis=0) / np.linalg.norm(B, axis=0)
     assert res.max() < 1e-9
+    x0[:, 0] = float('nan')                              # a guess that is not finite starts from zero too
+    X2, i2 = gmres_batched(lambda V: Ad @ V, Bd, x0=x0, guess=True, **kw)
+    assert i2 == i1 and torch.equal(torch.view_as_real(X2), torch.view_as_real(X1))
