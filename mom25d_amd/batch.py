@@ -118,12 +118,13 @@ class SweepBatch:
             cur.append(i)
         groups.append(cur)
         I_of, ok, self.iters = {}, {}, [None] * len(pts)
-        for grp in groups:
-            Y, okg, it = self._refine([Zs[i] for i in grp], [F.factors[i] for i in grp], [Bs[i] for i in grp],
-                                      [ks[i] for i in grp])
+        runs = [(grp,) + self._refine([Zs[i] for i in grp], [F.factors[i] for i in grp], [Bs[i] for i in grp],
+                                      [ks[i] for i in grp]) for grp in groups]
+        self._polish(runs, Zs, F.factors, Bs, ks)
+        for grp, Y, okc, rel, it in runs:
             first = np.concatenate([[0], np.cumsum([ks[i] for i in grp])[:-1]])
             for j, i in enumerate(grp):
-                ok[i] = okg[j]
+                ok[i] = bool(okc[first[j]:first[j] + ks[i]].all())
                 I_of[i] = ds[i][:, None] * Y[:Zs[i].shape[0], first[j]:first[j] + ks[i]]
                 self.iters[i] = it[first[j]:first[j] + ks[i]]
         torch.cuda.current_stream().synchronize()
@@ -146,8 +147,8 @@ class SweepBatch:
         return out
 
     def _refine(self, Zs, factors, Bs, ks):
-        """GMRES over the given points' columns side by side: (Y (n_max, k_total), converged per point, iterations
-        per column)."""
+        """GMRES over the given points' columns side by side: (Y (n_max, k_total), accepted per column, relative
+        residual per column, iterations per column)."""
         nmax = max(Z.shape[0] for Z in Zs)
         cols = _Cols(ks)
         B = torch.zeros((nmax, int(cols.k.sum())), dtype=dv.c128, device=Zs[0].device)
@@ -163,20 +164,46 @@ class SweepBatch:
         R = B - A(Y)
         rel = (torch.linalg.vector_norm(R, dim=0) /
                torch.clamp(torch.linalg.vector_norm(B, dim=0), min=1e-300)).cpu().numpy()
-        # answers accepted within 100 x tol, polished by one more cycle when still above POLISH x tol (the single
-        # path's rule), the others left to the single-point fallback
+        # answers accepted within 100 x tol (the single path's rule); the others are left to the single-point
+        # fallback, and the accepted ones still above POLISH x tol to _polish
         okc = np.isfinite(rel) & (rel <= 100 * tol)
-        more = np.flatnonzero(okc & (rel > POLISH * tol))
-        if len(more):
-            sub = torch.from_numpy(more).to(B.device)
-            pc = _Cols(np.bincount(cols.owner[more], minlength=len(ks)))
-            solve2 = dv.BatchSolve(factors, pc.active)
-            A2 = lambda V: dv.zgemm_nk_batched(Zs, V, pc.active)
-            M2 = lambda R: solve2(R, trans=1, subset=pc.active)
-            Y2, it2 = gmres_batched(A2, B[:, sub], M=M2, x0=Y[:, sub], rtol=tol, restart=MIXED_MAX_ITER, maxiter=1,
-                                    track=pc.track, stall_each=True)
-            Y[:, sub] = Y2
-            for c, n in zip(more, it2):
-                it[c] += n
-        ok = [bool(okc[cols.first[i]:cols.first[i] + ks[i]].all()) for i in range(len(ks))]
-        return Y, ok, it
+        return Y, okc, rel, it
+
+    def _polish(self, runs, Zs, factors, Bs, ks):
+        """The single path's polish for every group at once: one more GMRES cycle, from its answer, for each
+        accepted column still above POLISH x tol, the columns of all groups side by side (one batched solve and
+        product an iteration, not one a group). runs: [(group, Y, accepted, residual, iterations)], updated."""
+        tol = 1e-6
+        sel = []                                       # (run, point of the group, column of the point, block column)
+        for r, (grp, Y, okc, rel, it) in enumerate(runs):
+            first = np.concatenate([[0], np.cumsum([ks[i] for i in grp])[:-1]])
+            for j, i in enumerate(grp):
+                for c in range(ks[i]):
+                    b = first[j] + c
+                    if okc[b] and rel[b] > POLISH * tol:
+                        sel.append((r, j, c, b))
+        if not sel:
+            return
+        pts = sorted({(r, j) for r, j, _, _ in sel})
+        idx = {p: n for n, p in enumerate(pts)}
+        gi = [runs[r][0][j] for r, j in pts]          # the points' indices in the batch
+        cnt = np.bincount([idx[(r, j)] for r, j, _, _ in sel], minlength=len(pts))
+        sel.sort(key=lambda e: (idx[(e[0], e[1])], e[2]))
+        nmax = max(Zs[i].shape[0] for i in gi)
+        dev = Zs[gi[0]].device
+        B = torch.zeros((nmax, len(sel)), dtype=dv.c128, device=dev)
+        X0 = torch.zeros_like(B)
+        for q, (r, j, c, b) in enumerate(sel):
+            n = Zs[runs[r][0][j]].shape[0]
+            B[:n, q] = Bs[runs[r][0][j]][:, c]
+            X0[:n, q] = runs[r][1][:n, b]
+        pc = _Cols(cnt)
+        solve = dv.BatchSolve([factors[i] for i in gi], pc.active)
+        A = lambda V: dv.zgemm_nk_batched([Zs[i] for i in gi], V, pc.active)
+        M = lambda R: solve(R, trans=1, subset=pc.active)
+        Y2, it2 = gmres_batched(A, B, M=M, x0=X0, rtol=tol, restart=MIXED_MAX_ITER, maxiter=1, track=pc.track,
+                                stall_each=True)
+        for q, (r, j, c, b) in enumerate(sel):
+            n = Zs[runs[r][0][j]].shape[0]
+            runs[r][1][:n, b] = Y2[:n, q]
+            runs[r][4][b] += it2[q]
