@@ -34,10 +34,8 @@ __device__ __forceinline__ void zgemm_body(const double2 *__restrict__ Z, long n
         for (int c = 0; c < K; ++c) acc[r][c] = make_double2(0, 0);
     for (long j0 = 0; j0 < n; j0 += CH) {
         const long jn = n - j0 < CH ? n - j0 : CH;
-        for (int idx = tid; idx < CH * K; idx += 256) {
-            const int t = idx / K, c = idx % K;
-            xs[c][t] = (t < jn && c < kk) ? X[(j0 + t) * ldx + c] : make_double2(0, 0);
-        }
+        // the chunk's Z entries first (from HBM: the long wait), then its X values (K a thread, all in flight
+        // before the first is stored to LDS)
         double2 z[RPW][UJ];
 #pragma unroll
         for (int r = 0; r < RPW; ++r)
@@ -46,6 +44,17 @@ __device__ __forceinline__ void zgemm_body(const double2 *__restrict__ Z, long n
                 const long jj = lane + 64 * u;
                 z[r][u] = (i0 + r < n && jj < jn) ? Z[(i0 + r) * ld + j0 + jj] : make_double2(0, 0);
             }
+        double2 xv[CH * K / 256];
+#pragma unroll
+        for (int it = 0; it < CH * K / 256; ++it) {
+            const int idx = tid + it * 256, t = idx / K, c = idx % K;
+            xv[it] = (t < jn && c < kk) ? X[(j0 + t) * ldx + c] : make_double2(0, 0);
+        }
+#pragma unroll
+        for (int it = 0; it < CH * K / 256; ++it) {
+            const int idx = tid + it * 256;
+            xs[idx % K][idx / K] = xv[it];
+        }
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < UJ; ++u) {
